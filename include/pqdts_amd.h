@@ -1,0 +1,153 @@
+/*
+ * pqdts_amd -- C ABI of the MI355X (gfx950) two-metric projected truncated-Newton
+ * solver for quantum detector tomography:
+ *
+ *   min_Pi ||P - F Pi||_F^2 + gamma * sum_n sum_i (Pi[i,n] - Pi[i+1,n])^2,
+ *   every row of Pi (M x N) on the probability simplex.
+ *
+ * The reference (pc2/pqdts) is one Fortran program whose solver lives in
+ * `module basics` (pqdts.f90:24-1189) and in the minimisation loop of
+ * `program pqdts` (pqdts.f90:1908-2168). Its only in-process foreign call is
+ * simplexproj_condat_ (condat_simplexproj.c:60). This header exposes the same
+ * operators, one entry point per reference subroutine, on a device-resident
+ * problem context, plus the whole minimisation loop. The Fortran drop-in driver
+ * (pqdts_omp.x) binds these through iso_c_binding.
+ *
+ * Conventions
+ *   - every function returns 0 on success, or a nonzero code; pqdts_last_error()
+ *     then describes the failure (a HIP error, a bad argument, a missing GPU).
+ *   - host arrays are plain pointers owned by the caller; the library copies.
+ *   - M x N host arrays are row-major (C order, element (i,n) at i*N+n) when
+ *     layout == PQDTS_ROW_MAJOR, or column-major like the reference's X(M,N)
+ *     when layout == PQDTS_COL_MAJOR.
+ *   - COO index arrays are 0-based int32, exactly the data/[FP]_{row,col}.bin
+ *     files (pqdts.f90:1080-1107); duplicate (row,col) entries: the last wins
+ *     (pqdts.f90:1359-1363, 1533-1535).
+ *   - calls on one context are stream-ordered and not thread-safe; different
+ *     contexts are independent.
+ */
+#ifndef PQDTS_AMD_H
+#define PQDTS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pqdts_ctx pqdts_ctx;
+
+/* device-resident M x N buffers of a context (the reference's arrays) */
+enum pqdts_slot {
+  PQDTS_X = 0,     /* iterate X                    (pqdts.f90:1540)        */
+  PQDTS_G = 1,     /* gradient / CG residual, dLdX (pqdts.f90:1542)        */
+  PQDTS_HP = 2,    /* Hessian * direction, bd in the CG loop (2037)        */
+  PQDTS_XTMP = 3,  /* CG direction / line-search point, Xtmp (1548)        */
+  PQDTS_SN = 4,    /* Newton step sn (1550)                                */
+  PQDTS_Z = 5      /* preconditioned residual z (1552)                     */
+};
+
+enum pqdts_layout { PQDTS_ROW_MAJOR = 0, PQDTS_COL_MAJOR = 1 };
+
+/* Progress events of pqdts_solve, with the values the reference prints. */
+enum pqdts_event {
+  PQDTS_EV_ITER = 0,    /* oiter iiter L xmin conv ||der||^2 mode alpha dlalpha  (pqdts.f90:2131) */
+  PQDTS_EV_PCG = 1,     /* j sqrt(b2) eps b1 v2                                (pqdts.f90:2084) */
+  PQDTS_EV_LS = 2,      /* i L alpha df xmin                                   (pqdts.f90:429/474) */
+  PQDTS_EV_ACTIVE = 3,  /* nact M*N                                            (pqdts.f90:2002) */
+  PQDTS_EV_STOP1 = 4,   /* dlalpha                                             (pqdts.f90:1956) */
+  PQDTS_EV_ITER0 = 5,   /* L0 sqrt(||der||^2)                                  (pqdts.f90:1914) */
+  PQDTS_EV_LAMBDA = 6,  /* i lambda min(G+lambda)                              (pqdts.f90:1059) */
+  PQDTS_EV_STAGE = 7    /* oiter conv: end of a stage, before its output file  (pqdts.f90:2139) */
+};
+typedef void (*pqdts_event_cb)(void *user, int kind, const double *values, int nvalues);
+
+/* Message describing the last failure on this thread ("" if none). */
+const char *pqdts_last_error(void);
+
+/* Number of visible GPUs (0 when none); never fails. */
+int pqdts_device_count(void);
+
+/* Builds a problem on the current GPU from COO F (D x M) and P (D x N):
+ * the reference's F ingest (pqdts.f90:1318-1365), P densify (1527-1538) and
+ * work-array allocation (1540-1563). X starts at 0, maxr at -1, full space. */
+int pqdts_create(int64_t M, int64_t N, int64_t D,
+                 int64_t nF, const int32_t *F_row, const int32_t *F_col, const double *F_val,
+                 int64_t nP, const int32_t *P_row, const int32_t *P_col, const double *P_val,
+                 double gamma, pqdts_ctx **out);
+int pqdts_destroy(pqdts_ctx *ctx);
+
+/* host <-> device copies of a slot */
+int pqdts_put(pqdts_ctx *ctx, int slot, const double *host, int layout);
+int pqdts_get(pqdts_ctx *ctx, int slot, double *host, int layout);
+/* O = F*X as left by the last objective/gradient/HVP evaluation, D x N row-major */
+int pqdts_get_O(pqdts_ctx *ctx, double *host);
+/* maxr (pqdts.f90:93), 0-based column index per row or -1 */
+int pqdts_set_maxr(pqdts_ctx *ctx, const int32_t *maxr);
+int pqdts_get_maxr(pqdts_ctx *ctx, int32_t *maxr);
+/* active-set mask ik (pqdts.f90:92), M x N row-major */
+int pqdts_get_ik(pqdts_ctx *ctx, int8_t *ik);
+/* the reference's global tfullspace flag (pqdts.f90:97) */
+int pqdts_set_fullspace(pqdts_ctx *ctx, int full);
+int pqdts_get_fullspace(pqdts_ctx *ctx, int *full);
+
+/* ---- operators: one per reference subroutine --------------------------- */
+/* L (pqdts.f90:917-973): objective at slot x; reconstructs x's implicit column in
+ * place when not in full space. */
+int pqdts_L(pqdts_ctx *ctx, int x, double *val);
+/* dL (pqdts.f90:975-1032): gradient of slot x into slot g. */
+int pqdts_dL(pqdts_ctx *ctx, int x, int g);
+/* BF (pqdts.f90:246-342): Hessian times slot dd into slot bx. */
+int pqdts_BF(pqdts_ctx *ctx, int dd, int bx);
+/* Bdiag (pqdts.f90:208-244): Hessian diagonal; every column of the reference's
+ * M x N result is the same, so this returns the M-vector (diag may be NULL). */
+int pqdts_Bdiag(pqdts_ctx *ctx, double *diag);
+/* doproj (pqdts.f90:1120-1140): Euclidean projection of every row of slot x
+ * onto the simplex, the reference's simplexproj_condat_ per row. */
+int pqdts_doproj(pqdts_ctx *ctx, int x);
+/* doproj2 (pqdts.f90:1142-1164): clip negatives, rebuild the implicit column,
+ * xmin = min(0, min_i x[i,maxr(i)]). */
+int pqdts_doproj2(pqdts_ctx *ctx, int x, double *xmin);
+/* conv_test (pqdts.f90:1034-1067): KKT measure; sets full space as the reference does. */
+int pqdts_conv_test(pqdts_ctx *ctx, int x, int g, double *conv);
+/* pdot (pqdts.f90:589-614) */
+int pqdts_pdot(pqdts_ctx *ctx, int x, int y, double *val);
+/* BLAS-1 of the reference (pqdts.f90:485-587), on slots */
+int pqdts_paxpy(pqdts_ctx *ctx, double alpha, int x, int y);                /* y += alpha x        */
+int pqdts_pgaxpy(pqdts_ctx *ctx, double alpha, int x, double beta, int y);  /* y = beta y + alpha x */
+int pqdts_pgaxpy2(pqdts_ctx *ctx, double alpha, int x, double beta, int y, int z); /* z = beta y + alpha x */
+int pqdts_pset(pqdts_ctx *ctx, double alpha, int x);
+int pqdts_pcopy(pqdts_ctx *ctx, int x, int y);
+int pqdts_pscale(pqdts_ctx *ctx, double alpha, int x);
+int pqdts_pcopy_scale(pqdts_ctx *ctx, double alpha, int x, int y);
+/* reduced-space transform of X (pqdts.f90:1970-1980) */
+int pqdts_transform(pqdts_ctx *ctx);
+/* Line searches along slot SN from slot X, trial points in XTMP:
+ * Lalpha_min2 (pqdts.f90:394-438) and Lalpha_min_auglag (pqdts.f90:440-483). */
+int pqdts_lalpha_min2(pqdts_ctx *ctx, double *alpha, double *dlalpha);
+int pqdts_lalpha_min_auglag(pqdts_ctx *ctx, double *alpha, double *dlalpha);
+
+/* The minimisation (pqdts.f90:1908-2167) from the current X: mode 1 (projected
+ * gradient) or 2 (two-metric truncated Newton), stages start_stage..2, at most
+ * maxiter Newton iterations per stage, stop when conv < eps. `cb` (may be NULL)
+ * receives every progress event in the reference's print order; output >= 3
+ * also reports the per-row lambda lines. A PQDTS_EV_STAGE event closes each
+ * stage after its final conv_test; the callback may then call pqdts_get on the
+ * same context to write the per-stage file. *conv receives the last KKT value. */
+int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage, int output,
+                pqdts_event_cb cb, void *user, double *conv);
+
+/* Device-side timing of the named sections of the last solve (names as the
+ * reference's timers: "L", "dL", "BF", "Lalpha_min", ...). Returns the number of
+ * timers; fills up to `cap` entries. */
+int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *seconds);
+
+/* simplexproj_condat_ (condat_simplexproj.c:60) over many rows at once: rows of
+ * length n from y (host, row-major) projected onto the simplex into x (host).
+ * x must not alias y, as in the reference. */
+int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

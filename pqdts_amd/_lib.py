@@ -1,0 +1,277 @@
+"""ctypes binding of libpqdts_amd.so (the C ABI in include/pqdts_amd.h).
+
+The library is the product: every operator runs as HIP kernels on the GPU.
+There is no CPU fallback; loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpqdts_amd.so")
+
+SLOTS = {"X": 0, "G": 1, "HP": 2, "XTMP": 3, "SN": 4, "Z": 5}
+ROW_MAJOR, COL_MAJOR = 0, 1
+EV_ITER, EV_PCG, EV_LS, EV_ACTIVE, EV_STOP1, EV_ITER0, EV_LAMBDA, EV_STAGE = range(8)
+EVENT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int)
+
+# every symbol the header declares, for the load check
+EXPORTS = [
+    "pqdts_last_error", "pqdts_device_count", "pqdts_create", "pqdts_destroy", "pqdts_put", "pqdts_get",
+    "pqdts_get_O", "pqdts_set_maxr", "pqdts_get_maxr", "pqdts_get_ik", "pqdts_set_fullspace",
+    "pqdts_get_fullspace", "pqdts_L", "pqdts_dL", "pqdts_BF", "pqdts_Bdiag", "pqdts_doproj", "pqdts_doproj2",
+    "pqdts_conv_test", "pqdts_pdot", "pqdts_paxpy", "pqdts_pgaxpy", "pqdts_pgaxpy2", "pqdts_pset",
+    "pqdts_pcopy", "pqdts_pscale", "pqdts_pcopy_scale", "pqdts_transform", "pqdts_lalpha_min2",
+    "pqdts_lalpha_min_auglag", "pqdts_solve", "pqdts_timers", "pqdts_simplexproj_rows",
+]
+
+_lib = None
+
+
+class PqdtsError(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PqdtsError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    vp, d, i64, i32, dp = C.c_void_p, C.c_double, C.c_int64, C.c_int, C.POINTER(C.c_double)
+    i32p = C.POINTER(C.c_int32)
+    L.pqdts_last_error.restype = C.c_char_p
+    L.pqdts_device_count.restype = i32
+    L.pqdts_create.argtypes = [i64, i64, i64, i64, i32p, i32p, dp, i64, i32p, i32p, dp, d, C.POINTER(vp)]
+    L.pqdts_destroy.argtypes = [vp]
+    L.pqdts_put.argtypes = [vp, i32, dp, i32]
+    L.pqdts_get.argtypes = [vp, i32, dp, i32]
+    L.pqdts_get_O.argtypes = [vp, dp]
+    L.pqdts_set_maxr.argtypes = [vp, i32p]
+    L.pqdts_get_maxr.argtypes = [vp, i32p]
+    L.pqdts_get_ik.argtypes = [vp, C.POINTER(C.c_int8)]
+    L.pqdts_set_fullspace.argtypes = [vp, i32]
+    L.pqdts_get_fullspace.argtypes = [vp, C.POINTER(i32)]
+    L.pqdts_L.argtypes = [vp, i32, dp]
+    L.pqdts_dL.argtypes = [vp, i32, i32]
+    L.pqdts_BF.argtypes = [vp, i32, i32]
+    L.pqdts_Bdiag.argtypes = [vp, dp]
+    L.pqdts_doproj.argtypes = [vp, i32]
+    L.pqdts_doproj2.argtypes = [vp, i32, dp]
+    L.pqdts_conv_test.argtypes = [vp, i32, i32, dp]
+    L.pqdts_pdot.argtypes = [vp, i32, i32, dp]
+    L.pqdts_paxpy.argtypes = [vp, d, i32, i32]
+    L.pqdts_pgaxpy.argtypes = [vp, d, i32, d, i32]
+    L.pqdts_pgaxpy2.argtypes = [vp, d, i32, d, i32, i32]
+    L.pqdts_pset.argtypes = [vp, d, i32]
+    L.pqdts_pcopy.argtypes = [vp, i32, i32]
+    L.pqdts_pscale.argtypes = [vp, d, i32]
+    L.pqdts_pcopy_scale.argtypes = [vp, d, i32, i32]
+    L.pqdts_transform.argtypes = [vp]
+    L.pqdts_lalpha_min2.argtypes = [vp, dp, dp]
+    L.pqdts_lalpha_min_auglag.argtypes = [vp, dp, dp]
+    L.pqdts_solve.argtypes = [vp, i32, i32, d, i32, i32, EVENT_CB, vp, dp]
+    L.pqdts_timers.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
+    L.pqdts_simplexproj_rows.argtypes = [dp, dp, i64, i64]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().pqdts_last_error().decode(errors="replace")
+        raise PqdtsError(f"pqdts_amd error {rc}: {msg}")
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def device_count() -> int:
+    return int(load().pqdts_device_count())
+
+
+def simplexproj_rows(y: np.ndarray) -> np.ndarray:
+    """Projects every row of y onto the probability simplex on the GPU."""
+    y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+    x = np.empty_like(y)
+    _check(load().pqdts_simplexproj_rows(_p(y), _p(x), y.shape[0], y.shape[1]))
+    return x
+
+
+def _slot(s):
+    return SLOTS[s] if isinstance(s, str) else int(s)
+
+
+class Context:
+    """A device-resident problem: F (D x M) and P (D x N) as COO, gamma."""
+
+    def __init__(self, M, N, D, F_coo, P_coo, gamma):
+        L = load()
+        fr = np.ascontiguousarray(F_coo[0], np.int32)
+        fc = np.ascontiguousarray(F_coo[1], np.int32)
+        fv = np.ascontiguousarray(F_coo[2], np.float64)
+        pr = np.ascontiguousarray(P_coo[0], np.int32)
+        pc = np.ascontiguousarray(P_coo[1], np.int32)
+        pv = np.ascontiguousarray(P_coo[2], np.float64)
+        self.M, self.N, self.D = int(M), int(N), int(D)
+        h = C.c_void_p()
+        _check(L.pqdts_create(self.M, self.N, self.D, fv.size, _p(fr, C.c_int32), _p(fc, C.c_int32), _p(fv),
+                              pv.size, _p(pr, C.c_int32), _p(pc, C.c_int32), _p(pv), float(gamma), C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_problem(cls, prob):
+        import scipy.sparse as sps
+        F = sps.coo_matrix(prob.F)
+        P = sps.coo_matrix(prob.P)
+        return cls(prob.M, prob.N, prob.D, (F.row, F.col, F.data), (P.row, P.col, P.data), prob.gamma)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().pqdts_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def put(self, slot, arr, layout=ROW_MAJOR):
+        a = np.ascontiguousarray(arr, dtype=np.float64) if layout == ROW_MAJOR else np.asfortranarray(arr, np.float64)
+        assert a.size == self.M * self.N
+        _check(load().pqdts_put(self.h, _slot(slot), _p(a), layout))
+
+    def get(self, slot, layout=ROW_MAJOR) -> np.ndarray:
+        out = np.empty((self.M, self.N), dtype=np.float64, order="C" if layout == ROW_MAJOR else "F")
+        _check(load().pqdts_get(self.h, _slot(slot), _p(out), layout))
+        return out
+
+    def get_O(self):
+        out = np.empty((self.D, self.N))
+        _check(load().pqdts_get_O(self.h, _p(out)))
+        return out
+
+    @property
+    def maxr(self):
+        out = np.empty(self.M, np.int32)
+        _check(load().pqdts_get_maxr(self.h, _p(out, C.c_int32)))
+        return out
+
+    @maxr.setter
+    def maxr(self, v):
+        a = np.ascontiguousarray(v, np.int32)
+        _check(load().pqdts_set_maxr(self.h, _p(a, C.c_int32)))
+
+    @property
+    def ik(self):
+        out = np.empty((self.M, self.N), np.int8)
+        _check(load().pqdts_get_ik(self.h, _p(out, C.c_int8)))
+        return out
+
+    @property
+    def fullspace(self) -> bool:
+        v = C.c_int()
+        _check(load().pqdts_get_fullspace(self.h, C.byref(v)))
+        return bool(v.value)
+
+    @fullspace.setter
+    def fullspace(self, full: bool):
+        _check(load().pqdts_set_fullspace(self.h, int(bool(full))))
+
+    def _scalar(self, fn, *args):
+        v = C.c_double()
+        _check(fn(self.h, *args, C.byref(v)))
+        return v.value
+
+    def L(self, x="X"):
+        return self._scalar(load().pqdts_L, _slot(x))
+
+    def dL(self, x="X", g="G"):
+        _check(load().pqdts_dL(self.h, _slot(x), _slot(g)))
+
+    def BF(self, dd="XTMP", bx="HP"):
+        _check(load().pqdts_BF(self.h, _slot(dd), _slot(bx)))
+
+    def Bdiag(self):
+        out = np.empty(self.M)
+        _check(load().pqdts_Bdiag(self.h, _p(out)))
+        return out
+
+    def doproj(self, x="X"):
+        _check(load().pqdts_doproj(self.h, _slot(x)))
+
+    def doproj2(self, x="X"):
+        return self._scalar(load().pqdts_doproj2, _slot(x))
+
+    def conv_test(self, x="X", g="G"):
+        return self._scalar(load().pqdts_conv_test, _slot(x), _slot(g))
+
+    def pdot(self, x, y):
+        return self._scalar(load().pqdts_pdot, _slot(x), _slot(y))
+
+    def paxpy(self, a, x, y):
+        _check(load().pqdts_paxpy(self.h, float(a), _slot(x), _slot(y)))
+
+    def pgaxpy(self, a, x, b, y):
+        _check(load().pqdts_pgaxpy(self.h, float(a), _slot(x), float(b), _slot(y)))
+
+    def pgaxpy2(self, a, x, b, y, z):
+        _check(load().pqdts_pgaxpy2(self.h, float(a), _slot(x), float(b), _slot(y), _slot(z)))
+
+    def pset(self, a, x):
+        _check(load().pqdts_pset(self.h, float(a), _slot(x)))
+
+    def pcopy(self, x, y):
+        _check(load().pqdts_pcopy(self.h, _slot(x), _slot(y)))
+
+    def pscale(self, a, x):
+        _check(load().pqdts_pscale(self.h, float(a), _slot(x)))
+
+    def pcopy_scale(self, a, x, y):
+        _check(load().pqdts_pcopy_scale(self.h, float(a), _slot(x), _slot(y)))
+
+    def transform(self):
+        _check(load().pqdts_transform(self.h))
+
+    def lalpha_min2(self):
+        a, dl = C.c_double(), C.c_double()
+        _check(load().pqdts_lalpha_min2(self.h, C.byref(a), C.byref(dl)))
+        return a.value, dl.value
+
+    def lalpha_min_auglag(self):
+        a, dl = C.c_double(), C.c_double()
+        _check(load().pqdts_lalpha_min_auglag(self.h, C.byref(a), C.byref(dl)))
+        return a.value, dl.value
+
+    def solve(self, mode=2, maxiter=200, eps=1e-6, start_stage=1, output=0, on_event=None):
+        """Runs the minimisation from the current X; returns (conv, events)."""
+        events = []
+
+        def cb(_u, kind, v, nv):
+            ev = (kind, tuple(v[i] for i in range(nv)))
+            events.append(ev)
+            if on_event is not None:
+                on_event(self, *ev)
+
+        fn = EVENT_CB(cb)
+        conv = C.c_double()
+        _check(load().pqdts_solve(self.h, int(mode), int(maxiter), float(eps), int(start_stage), int(output), fn,
+                                  None, C.byref(conv)))
+        return conv.value, events
+
+    def timers(self):
+        L = load()
+        n = L.pqdts_timers(self.h, 0, None, None, None)
+        if n <= 0:
+            return {}
+        names = (C.c_char * 24 * n)()
+        counts = (C.c_int * n)()
+        secs = (C.c_double * n)()
+        L.pqdts_timers(self.h, n, C.cast(names, C.c_void_p), counts, secs)
+        return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], secs[i]) for i in range(n)}

@@ -1,0 +1,931 @@
+// Host side of pqdts_amd: problem context, the C ABI declared in
+// include/pqdts_amd.h, and the minimisation loop (pqdts.f90:1908-2167) driving
+// the kernels of kernels.hip on one HIP stream.
+#include "../../include/pqdts_amd.h"
+#include "kernels.hip"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace pq;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw Fail((int)e, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+void require(bool ok, const std::string &msg) {
+  if (!ok) throw Fail(-1, msg);
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    g_err.clear();
+    f();
+    return 0;
+  } catch (const Fail &e) {
+    g_err = e.what();
+    return e.code ? e.code : -1;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// tile heights of the two products
+constexpr int RD = 32;   // F rows per block of k_spmm_rows
+constexpr int RK = 32;   // G rows per block of k_spmmT_cols
+constexpr int KC = 64;   // staged chunk length
+
+// device scalar slots
+enum {
+  S_LA = 0, S_LB, S_XMIN, S_NACT, S_ND, S_B1, S_V2, S_RZ0, S_RZ1, S_RR, S_CONV, S_DOT, S_NSCAL = 16
+};
+
+// partial-sum regions (each PCAP doubles)
+enum { R_A = 0, R_B, R_C, R_D, R_NREG };
+
+inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+struct Timer {
+  int count = 0;
+  double sec = 0.0;
+  std::chrono::steady_clock::time_point t0;
+  bool running = false;
+};
+
+}  // namespace
+
+struct pqdts_ctx {
+  int M = 0, N = 0, D = 0;
+  int64_t MN = 0, DN = 0;
+  double gamma = 0.0;
+  int device = 0;
+  hipStream_t s = nullptr;
+  // F as dense row spans
+  int *flo = nullptr, *flen = nullptr;
+  int64_t *foff = nullptr;
+  double *fval = nullptr;
+  uint8_t *fhas = nullptr;
+  int nrt = 0, nct = 0;
+  int *rt_lo = nullptr, *rt_hi = nullptr, *ct_lo = nullptr, *ct_hi = nullptr;
+  int *kd_lo = nullptr, *kd_hi = nullptr;
+  double *P = nullptr, *O = nullptr;
+  double *slot[6] = {};
+  double *diag_full = nullptr, *diag_red = nullptr;
+  int *maxr = nullptr;
+  uint8_t *ik = nullptr;
+  double *ev = nullptr, *gm = nullptr, *lam = nullptr, *lmin = nullptr;
+  double *part = nullptr;
+  int pcap = 0;
+  double *scal = nullptr, *hscal = nullptr;
+  bool full = true;
+  bool maxr_valid = false;  // every maxr[i] in [0, N)
+  int mode = 2;
+  bool sync_timers = false;
+  std::map<std::string, Timer> timers;
+  std::vector<std::string> timer_order;
+  pqdts_event_cb cb = nullptr;
+  void *user = nullptr;
+
+  double *reg(int r) { return part + (int64_t)r * pcap; }
+  const double *diag() const { return full ? diag_full : diag_red; }
+  int row_blocks() const { return cdiv(M, WAVES_PER_BLOCK); }
+
+  void tstart(const char *name) {
+    auto it = timers.find(name);
+    if (it == timers.end()) {
+      it = timers.emplace(name, Timer{}).first;
+      timer_order.push_back(name);
+    }
+    it->second.count++;
+    it->second.running = true;
+    it->second.t0 = std::chrono::steady_clock::now();
+  }
+  void tstop(const char *name) {
+    if (sync_timers) (void)hipStreamSynchronize(s);
+    auto &t = timers[name];
+    t.sec += std::chrono::duration<double>(std::chrono::steady_clock::now() - t.t0).count();
+    t.running = false;
+  }
+  void emit(int kind, std::initializer_list<double> v) {
+    if (cb) {
+      std::vector<double> tmp(v);
+      cb(user, kind, tmp.data(), (int)tmp.size());
+    }
+  }
+  // copy all device scalars to the host and wait for the stream
+  void fetch() {
+    HIPC(hipMemcpyAsync(hscal, scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+  }
+  void reduce(int region, int n, int dst, bool is_min = false) {
+    k_reduce<<<1, BLOCK, 0, s>>>(reg(region), n, scal, dst, is_min ? 1 : 0);
+  }
+};
+
+namespace {
+
+struct Scope {
+  pqdts_ctx *c;
+  const char *n;
+  Scope(pqdts_ctx *c_, const char *n_) : c(c_), n(n_) { c->tstart(n); }
+  ~Scope() { c->tstop(n); }
+};
+
+double *slot_ptr(pqdts_ctx *c, int slot) {
+  require(slot >= 0 && slot < 6, "invalid slot id " + std::to_string(slot));
+  return c->slot[slot];
+}
+
+template <class T>
+T *dalloc(int64_t n) {
+  void *p = nullptr;
+  HIPC(hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+  return (T *)p;
+}
+
+// ---------------------------------------------------------------------------
+// operators
+// ---------------------------------------------------------------------------
+
+// Reduced-space kernels index rows by maxr; refuse to launch them otherwise.
+void need_maxr(pqdts_ctx *c) {
+  require(c->maxr_valid, "reduced space needs a valid maxr (call pqdts_transform or pqdts_set_maxr)");
+}
+
+void op_reconstruct(pqdts_ctx *c, double *X) {
+  need_maxr(c);
+  k_reconstruct<<<c->row_blocks(), BLOCK, 0, c->s>>>(X, c->M, c->N, c->maxr);
+}
+
+// O = F V; returns the number of LOSS partials written to region R_A
+int op_spmm(pqdts_ctx *c, const double *V, bool expand, bool loss, bool store) {
+  dim3 grid(cdiv(c->N, BLOCK), c->nrt);
+  double *part = c->reg(R_A);
+#define SPMM_ARGS                                                                                      \
+  V, c->N, c->D, c->flo, c->flen, c->foff, c->fval, c->fhas, c->rt_lo, c->rt_hi, c->maxr, c->ev, c->P, \
+      c->O, part
+  if (expand)
+    k_spmm_rows<RD, KC, true, false, true><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
+  else if (loss && store)
+    k_spmm_rows<RD, KC, false, true, true><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
+  else if (loss)
+    k_spmm_rows<RD, KC, false, true, false><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
+  else
+    k_spmm_rows<RD, KC, false, false, true><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
+#undef SPMM_ARGS
+  HIPC(hipGetLastError());
+  return grid.x * grid.y;
+}
+
+// G = 2 F^T W + stencil (+ fold / mask / dot); returns #DOT partials in R_B
+int op_spmmT(pqdts_ctx *c, int mode, const double *E, const double *dd, double *G, bool mask, bool dot) {
+  const bool red = !c->full;
+  if (red) need_maxr(c);
+  if (red) {
+    if (mode == T_GRAD)
+      k_foldval<T_GRAD><<<c->row_blocks(), BLOCK, 0, c->s>>>(c->O, c->P, c->N, c->M, c->flo, c->flen, c->foff,
+                                                            c->fval, c->kd_lo, c->kd_hi, c->gamma, E, c->maxr,
+                                                            c->ev, c->gm);
+    else
+      k_foldval<T_HVP><<<c->row_blocks(), BLOCK, 0, c->s>>>(c->O, c->P, c->N, c->M, c->flo, c->flen, c->foff,
+                                                           c->fval, c->kd_lo, c->kd_hi, c->gamma, E, c->maxr,
+                                                           c->ev, c->gm);
+  }
+  dim3 grid(cdiv(c->N, BLOCK), c->nct);
+  double *part = c->reg(R_B);
+#define SPMMT_ARGS                                                                                        \
+  c->O, c->P, c->N, c->M, c->D, c->flo, c->flen, c->foff, c->fval, c->ct_lo, c->ct_hi, c->gamma, E, c->maxr, \
+      c->ev, c->gm, c->ik, dd, G, part
+  if (mode == T_GRAD) {
+    if (red)
+      k_spmmT_cols<RK, KC, T_GRAD, true, false, false><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+    else
+      k_spmmT_cols<RK, KC, T_GRAD, false, false, false><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+  } else {
+    if (red) {
+      if (mask && dot)
+        k_spmmT_cols<RK, KC, T_HVP, true, true, true><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+      else
+        k_spmmT_cols<RK, KC, T_HVP, true, false, false><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+    } else {
+      if (mask && dot)
+        k_spmmT_cols<RK, KC, T_HVP, false, true, true><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+      else
+        k_spmmT_cols<RK, KC, T_HVP, false, false, false><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+    }
+  }
+#undef SPMMT_ARGS
+  HIPC(hipGetLastError());
+  return grid.x * grid.y;
+}
+
+// objective (pqdts.f90:917-973); `recon` = apply the reduced-space prep
+double op_L(pqdts_ctx *c, double *X, bool recon = true, bool fetch = true) {
+  Scope t(c, "L");
+  if (!c->full && recon) op_reconstruct(c, X);
+  const int na = op_spmm(c, X, false, true, false);
+  c->reduce(R_A, na, S_LA);
+  if (c->gamma != 0.0) {
+    const int nb = std::min(cdiv((int64_t)(c->M - 1) * c->N, BLOCK), c->pcap);
+    if (nb > 0) {
+      k_smooth<<<nb, BLOCK, 0, c->s>>>(X, c->M, c->N, c->gamma, c->reg(R_B));
+      c->reduce(R_B, nb, S_LB);
+    } else {
+      HIPC(hipMemsetAsync(c->scal + S_LB, 0, sizeof(double), c->s));
+    }
+  } else {
+    HIPC(hipMemsetAsync(c->scal + S_LB, 0, sizeof(double), c->s));
+  }
+  if (!fetch) return 0.0;
+  c->fetch();
+  return c->hscal[S_LA] + c->hscal[S_LB];
+}
+
+// gradient (pqdts.f90:975-1032)
+void op_dL(pqdts_ctx *c, double *X, double *G) {
+  Scope t(c, "dL");
+  if (!c->full) op_reconstruct(c, X);
+  op_spmm(c, X, false, false, true);
+  op_spmmT(c, T_GRAD, X, nullptr, G, false, false);
+}
+
+// Hessian times dd (pqdts.f90:246-342). `have_ev`: ev already holds the
+// expansion of dd (computed by the CG kernels).
+int op_BF(pqdts_ctx *c, const double *dd, double *BX, bool have_ev, bool mask, bool dot) {
+  Scope t(c, "BF");
+  if (!c->full) need_maxr(c);
+  if (!c->full && !have_ev)
+    k_others<<<c->row_blocks(), BLOCK, 0, c->s>>>(dd, c->M, c->N, c->maxr, c->ev);
+  op_spmm(c, dd, !c->full, false, true);
+  return op_spmmT(c, T_HVP, dd, dd, BX, mask, dot);
+}
+
+double op_pdot(pqdts_ctx *c, const double *x, const double *y) {
+  Scope t(c, "pdot");
+  const int nb = std::min(cdiv(c->MN, BLOCK), std::min(c->pcap, 1024));
+  k_dot<<<nb, BLOCK, 0, c->s>>>(x, y, c->MN, c->reg(R_A));
+  c->reduce(R_A, nb, S_DOT);
+  c->fetch();
+  return c->hscal[S_DOT];
+}
+
+void op_doproj(pqdts_ctx *c, double *X) {
+  Scope t(c, "doproj");
+  k_simplex_rows<<<c->row_blocks(), BLOCK, 0, c->s>>>(X, nullptr, nullptr, 0.0, c->M, c->N, 0);
+  HIPC(hipGetLastError());
+}
+
+double op_doproj2(pqdts_ctx *c, double *X, bool fetch = true) {
+  Scope t(c, "doproj2");
+  require(!c->full, "doproj2 needs the reduced space (maxr set)");
+  need_maxr(c);
+  const int nb = c->row_blocks();
+  k_clip_rows<<<nb, BLOCK, 0, c->s>>>(X, nullptr, nullptr, 0.0, c->M, c->N, c->maxr, 0, 0, c->reg(R_C));
+  c->reduce(R_C, nb, S_XMIN, true);
+  if (!fetch) return 0.0;
+  c->fetch();
+  return c->hscal[S_XMIN];
+}
+
+// conv_test (pqdts.f90:1034-1067)
+double op_conv(pqdts_ctx *c, double *X, double *G, bool print) {
+  if (!c->full) op_reconstruct(c, X);
+  c->full = true;
+  op_dL(c, X, G);
+  const int nb = c->row_blocks();
+  k_conv_rows<<<nb, BLOCK, 0, c->s>>>(X, G, c->M, c->N, print ? c->lam : nullptr, print ? c->lmin : nullptr,
+                                      c->reg(R_C));
+  c->reduce(R_C, nb, S_CONV);
+  c->fetch();
+  if (print && c->cb) {
+    std::vector<double> lam(c->M), lmin(c->M);
+    HIPC(hipMemcpy(lam.data(), c->lam, sizeof(double) * c->M, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(lmin.data(), c->lmin, sizeof(double) * c->M, hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->M; i++) c->emit(PQDTS_EV_LAMBDA, {(double)(i + 1), lam[i], lmin[i]});
+  }
+  return std::sqrt(c->hscal[S_CONV] / ((double)c->N * (double)c->M));
+}
+
+// Objective at the trial point X + a*SN after projection (both line searches).
+// Returns L; *xmin receives the doproj2 minimum (0 for Condat).
+double trial_L(pqdts_ctx *c, double a, bool condat, double *xmin) {
+  double *T = c->slot[PQDTS_XTMP];
+  const int nb = c->row_blocks();
+  if (condat) {
+    Scope t(c, "doproj");
+    k_simplex_rows<<<nb, BLOCK, 0, c->s>>>(T, c->slot[PQDTS_X], c->slot[PQDTS_SN], a, c->M, c->N, 1);
+  } else {
+    Scope t(c, "doproj2");
+    k_clip_rows<<<nb, BLOCK, 0, c->s>>>(T, c->slot[PQDTS_X], c->slot[PQDTS_SN], a, c->M, c->N, c->maxr, 1, 1,
+                                        c->reg(R_C));
+    c->reduce(R_C, nb, S_XMIN, true);
+  }
+  HIPC(hipGetLastError());
+  // the reduced-space prep of L was applied inside k_clip_rows
+  double L = op_L(c, T, false, true);
+  *xmin = condat ? 0.0 : c->hscal[S_XMIN];
+  return L;
+}
+
+// Lalpha_min2 (pqdts.f90:394-438) and Lalpha_min_auglag (pqdts.f90:440-483)
+void op_linesearch(pqdts_ctx *c, bool condat, double *alpha_out, double *dlalpha) {
+  Scope t(c, "Lalpha_min");
+  if (!condat) {
+    require(!c->full, "Lalpha_min2 needs the reduced space (maxr set)");
+    need_maxr(c);
+  }
+  const double dx = (double)1e-5f;        // pqdts.f90:404
+  const double feas = (double)-1e-12f;    // pqdts.f90:424
+  double xmin;
+  const double L0 = trial_L(c, 0.0, condat, &xmin);
+  double Lp = trial_L(c, 0.0 + dx, condat, &xmin);
+  const double df = (Lp - L0) / dx;
+  *dlalpha = df;
+  double alpha = 1.0, lm = std::numeric_limits<double>::max(), alphamin = 0.0;
+  for (int i = 1; i <= 100; i++) {
+    Lp = trial_L(c, alpha, condat, &xmin);
+    const bool feasible = condat || xmin >= feas;
+    if (Lp < lm && feasible) {
+      lm = Lp;
+      alphamin = alpha;
+    }
+    c->emit(PQDTS_EV_LS, {(double)i, Lp, alpha, df, xmin});
+    if (Lp > L0 + 0.1 * alpha * df || !feasible)
+      alpha = alpha * 0.75;
+    else
+      break;
+  }
+  *alpha_out = alphamin;
+}
+
+void op_axpby(pqdts_ctx *c, double *z, const double *x, const double *y, double alpha, double beta, int kind) {
+  const int nb = std::max(1, std::min(cdiv(c->MN, BLOCK), 4096));
+  k_axpby<<<nb, BLOCK, 0, c->s>>>(z, x, y, alpha, beta, c->MN, kind);
+  HIPC(hipGetLastError());
+}
+
+void op_transform(pqdts_ctx *c, bool rebuild) {
+  k_transform<<<c->row_blocks(), BLOCK, 0, c->s>>>(c->slot[PQDTS_X], c->M, c->N, c->maxr, rebuild ? 1 : 0);
+  HIPC(hipGetLastError());
+  c->maxr_valid = true;
+}
+
+void fill_maxr(pqdts_ctx *c, int v) {
+  std::vector<int> h(c->M, v);
+  HIPC(hipMemcpyAsync(c->maxr, h.data(), sizeof(int) * c->M, hipMemcpyHostToDevice, c->s));
+  HIPC(hipStreamSynchronize(c->s));
+  c->maxr_valid = v >= 0 && v < c->N;
+}
+
+// ---------------------------------------------------------------------------
+// The minimisation loop (pqdts.f90:1908-2167)
+// ---------------------------------------------------------------------------
+double solve(pqdts_ctx *c, int mode, int maxiter, double conv_thres, int start_stage, int output) {
+  Scope tmin(c, "min");
+  double *X = c->slot[PQDTS_X], *G = c->slot[PQDTS_G], *HP = c->slot[PQDTS_HP];
+  double *XT = c->slot[PQDTS_XTMP], *SN = c->slot[PQDTS_SN], *Z = c->slot[PQDTS_Z];
+  const int M = c->M, N = c->N;
+  const int64_t MN = c->MN;
+  c->mode = mode;
+  double conv = 0.0, alpha = 0.0, dlalpha, norm_der, L0, xmin;
+
+  L0 = op_L(c, X);
+  op_dL(c, X, G);
+  norm_der = op_pdot(c, G, G);
+  c->emit(PQDTS_EV_ITER0, {L0, std::sqrt(norm_der)});
+
+  for (int oiter = start_stage; oiter <= 2; oiter++) {
+    dlalpha = -std::numeric_limits<double>::max();
+    for (int iiter = 1; iiter <= maxiter; iiter++) {
+      if (mode == 1) {
+        // projected gradient, pqdts.f90:1934-1951
+        c->full = false;
+        op_transform(c, false);
+        op_dL(c, X, G);
+        op_axpby(c, SN, G, nullptr, -1.0, 0.0, 2);
+        op_linesearch(c, false, &alpha, &dlalpha);
+      } else {
+        if ((std::fabs(dlalpha) < (double)1e-4f || dlalpha > 0) && oiter == 1) {
+          c->emit(PQDTS_EV_STOP1, {dlalpha});
+          break;
+        }
+        c->full = (oiter == 1);
+        if (!c->full)
+          op_transform(c, true);
+        else
+          fill_maxr(c, -1);
+        op_dL(c, X, G);
+        const bool red = !c->full;
+        if (red) need_maxr(c);
+        const int nb = c->row_blocks();
+        {
+          Scope t(c, "active_dLdX");
+          k_cg_init<<<nb, BLOCK, 0, c->s>>>(X, G, c->ik, Z, XT, SN, c->diag(), M, N, c->maxr, c->ev, red ? 1 : 0,
+                                            1, c->reg(R_A), c->reg(R_B), c->reg(R_C));
+          HIPC(hipGetLastError());
+          c->reduce(R_A, nb, S_NACT);
+          c->reduce(R_B, nb, S_ND);
+          c->reduce(R_C, nb, S_B1);
+          c->fetch();
+        }
+        c->emit(PQDTS_EV_ACTIVE, {c->hscal[S_NACT], (double)MN});
+        norm_der = c->hscal[S_ND];
+        const double eps = std::fmin(0.5, std::sqrt(std::sqrt(norm_der))) * std::sqrt(norm_der);
+        int i_b1 = S_B1;
+        for (int j = 1; j <= 50; j++) {
+          const int ndot = op_BF(c, XT, HP, true, true, true);
+          Scope t(c, "CG");
+          c->reduce(R_B, ndot, S_V2);
+          const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
+          k_cg_update<<<nb, BLOCK, 0, c->s>>>(SN, XT, G, HP, Z, c->diag(), M, N, c->scal, i_b1, S_V2, c->reg(R_A),
+                                              c->reg(R_C));
+          HIPC(hipGetLastError());
+          c->reduce(R_A, nb, i_rz);
+          c->reduce(R_C, nb, S_RR);
+          k_cg_dir<<<nb, BLOCK, 0, c->s>>>(XT, Z, M, N, c->scal, i_b1, i_rz, c->maxr, c->ev, red ? 1 : 0);
+          HIPC(hipGetLastError());
+          c->fetch();
+          const double b1 = c->hscal[i_b1], v2 = c->hscal[i_rz], b2 = c->hscal[S_RR];
+          c->emit(PQDTS_EV_PCG, {(double)j, std::sqrt(b2), eps, b1, v2});
+          if (std::sqrt(b2) <= eps) break;
+          i_b1 = i_rz;
+        }
+        // two-metric diagonal step, pqdts.f90:2092-2104
+        op_dL(c, X, G);
+        k_twometric<<<nb, BLOCK, 0, c->s>>>(SN, G, c->ik, c->diag(), M, N, c->maxr);
+        HIPC(hipGetLastError());
+        op_linesearch(c, c->full, &alpha, &dlalpha);
+      }
+      // step, projection and re-evaluation, pqdts.f90:2111-2128
+      if (alpha != 0.0) op_axpby(c, X, SN, X, alpha, 0.0, 1);
+      xmin = 0.0;
+      if (c->full)
+        op_doproj(c, X);
+      else
+        xmin = op_doproj2(c, X);
+      L0 = op_L(c, X);
+      op_dL(c, X, G);
+      norm_der = op_pdot(c, G, G);
+      conv = op_conv(c, X, G, false);
+      c->emit(PQDTS_EV_ITER,
+              {(double)oiter, (double)iiter, L0, xmin, conv, std::sqrt(norm_der), (double)mode, alpha, dlalpha});
+      if (conv < conv_thres) break;
+    }
+    conv = op_conv(c, X, G, output >= 3);
+    c->emit(PQDTS_EV_STAGE, {(double)oiter, conv});
+  }
+  HIPC(hipStreamSynchronize(c->s));
+  return conv;
+}
+
+void destroy_ctx(pqdts_ctx *c) {
+  if (!c) return;
+  void *ptrs[] = {c->flo,  c->flen, c->foff, c->fval,      c->fhas,     c->rt_lo, c->rt_hi, c->ct_lo,
+                  c->ct_hi, c->kd_lo, c->kd_hi, c->P,        c->O,        c->diag_full, c->diag_red, c->maxr,
+                  c->ik,   c->ev,   c->gm,   c->lam,       c->lmin,     c->part,  c->scal};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (double *p : c->slot)
+    if (p) (void)hipFree(p);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+}
+
+pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *Fr, const int32_t *Fc,
+                      const double *Fv, int64_t nP, const int32_t *Pr, const int32_t *Pc, const double *Pv,
+                      double gamma) {
+  require(M >= 1 && N >= 1 && D >= 1, "need M >= 1, N >= 1, D >= 1");
+  require(M * N < (int64_t)INT32_MAX * 8 && M < INT32_MAX && N < INT32_MAX && D < INT32_MAX, "problem too large");
+  require(nF >= 0 && nP >= 0, "negative nnz");
+  require(nF == 0 || (Fr && Fc && Fv), "null F arrays");
+  require(nP == 0 || (Pr && Pc && Pv), "null P arrays");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Fail(-2, "no GPU visible to pqdts_amd");
+
+  // F ingest: O(nnz) span build with zero fill and last-wins duplicates
+  // (same result as the O(D*nnz) scan of pqdts.f90:1328-1364)
+  std::vector<int> lo(D, INT32_MAX), hi(D, -1), len(D, 0);
+  std::vector<uint8_t> has(D, 0);
+  for (int64_t e = 0; e < nF; e++) {
+    const int d = Fr[e], k = Fc[e];
+    require(d >= 0 && d < D && k >= 0 && k < M, "F index out of range at entry " + std::to_string(e));
+    lo[d] = std::min(lo[d], k);
+    hi[d] = std::max(hi[d], k);
+    has[d] = 1;
+  }
+  std::vector<int64_t> off(D + 1, 0);
+  for (int d = 0; d < D; d++) {
+    if (!has[d]) lo[d] = 0;
+    len[d] = has[d] ? hi[d] - lo[d] + 1 : 0;
+    off[d + 1] = off[d] + len[d];
+  }
+  std::vector<double> val(off[D], 0.0);
+  for (int64_t e = 0; e < nF; e++) val[off[Fr[e]] + (Fc[e] - lo[Fr[e]])] = Fv[e];
+  // tiles of k_spmm_rows: union k-range of RD rows
+  const int nrt = cdiv(D, RD);
+  std::vector<int> rtlo(nrt, 0), rthi(nrt, -1);
+  for (int t = 0; t < nrt; t++) {
+    int a = INT32_MAX, b = -1;
+    for (int d = t * RD; d < std::min<int64_t>(D, (int64_t)(t + 1) * RD); d++)
+      if (has[d]) {
+        a = std::min(a, lo[d]);
+        b = std::max(b, lo[d] + len[d] - 1);
+      }
+    if (b >= 0) {
+      rtlo[t] = a;
+      rthi[t] = b;
+    }
+  }
+  // per-k d-range and tiles of k_spmmT_cols
+  std::vector<int> kdlo(M, INT32_MAX), kdhi(M, -1);
+  for (int d = 0; d < D; d++)
+    for (int k = lo[d]; k < lo[d] + len[d]; k++) {
+      kdlo[k] = std::min(kdlo[k], d);
+      kdhi[k] = std::max(kdhi[k], d);
+    }
+  for (int k = 0; k < M; k++)
+    if (kdhi[k] < 0) kdlo[k] = 0;
+  const int nct = cdiv(M, RK);
+  std::vector<int> ctlo(nct, 0), cthi(nct, -1);
+  for (int t = 0; t < nct; t++) {
+    int a = INT32_MAX, b = -1;
+    for (int k = t * RK; k < std::min<int64_t>(M, (int64_t)(t + 1) * RK); k++)
+      if (kdhi[k] >= 0) {
+        a = std::min(a, kdlo[k]);
+        b = std::max(b, kdhi[k]);
+      }
+    if (b >= 0) {
+      ctlo[t] = a;
+      cthi[t] = b;
+    }
+  }
+  // P densify (pqdts.f90:1527-1535), last duplicate wins
+  std::vector<double> Ph((size_t)(D * N), 0.0);
+  for (int64_t e = 0; e < nP; e++) {
+    require(Pr[e] >= 0 && Pr[e] < D && Pc[e] >= 0 && Pc[e] < N, "P index out of range at entry " + std::to_string(e));
+    Ph[(size_t)Pr[e] * N + Pc[e]] = Pv[e];
+  }
+
+  pqdts_ctx *c = new pqdts_ctx();
+  try {
+    c->M = (int)M;
+    c->N = (int)N;
+    c->D = (int)D;
+    c->MN = M * N;
+    c->DN = D * N;
+    c->gamma = gamma;
+    c->sync_timers = getenv("PQDTS_SYNC_TIMERS") && atoi(getenv("PQDTS_SYNC_TIMERS")) != 0;
+    HIPC(hipGetDevice(&c->device));
+    HIPC(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    c->nrt = nrt;
+    c->nct = nct;
+    c->flo = dalloc<int>(D);
+    c->flen = dalloc<int>(D);
+    c->foff = dalloc<int64_t>(D);
+    c->fval = dalloc<double>(off[D]);
+    c->fhas = dalloc<uint8_t>(D);
+    c->rt_lo = dalloc<int>(nrt);
+    c->rt_hi = dalloc<int>(nrt);
+    c->ct_lo = dalloc<int>(nct);
+    c->ct_hi = dalloc<int>(nct);
+    c->kd_lo = dalloc<int>(M);
+    c->kd_hi = dalloc<int>(M);
+    c->P = dalloc<double>(c->DN);
+    c->O = dalloc<double>(c->DN);
+    for (int s = 0; s < 6; s++) c->slot[s] = dalloc<double>(c->MN);
+    c->diag_full = dalloc<double>(M);
+    c->diag_red = dalloc<double>(M);
+    c->maxr = dalloc<int>(M);
+    c->ik = dalloc<uint8_t>(c->MN);
+    c->ev = dalloc<double>(M);
+    c->gm = dalloc<double>(M);
+    c->lam = dalloc<double>(M);
+    c->lmin = dalloc<double>(M);
+    const int gx = cdiv(N, BLOCK);
+    c->pcap = std::max({gx * nrt, gx * nct, cdiv(M, WAVES_PER_BLOCK), 4096});
+    c->part = dalloc<double>((int64_t)c->pcap * R_NREG);
+    c->scal = dalloc<double>(S_NSCAL);
+    HIPC(hipHostMalloc((void **)&c->hscal, sizeof(double) * S_NSCAL, hipHostMallocDefault));
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+      if (bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->s));
+    };
+    up(c->flo, lo.data(), sizeof(int) * D);
+    up(c->flen, len.data(), sizeof(int) * D);
+    up(c->foff, off.data(), sizeof(int64_t) * D);
+    up(c->fval, val.data(), sizeof(double) * off[D]);
+    up(c->fhas, has.data(), D);
+    up(c->rt_lo, rtlo.data(), sizeof(int) * nrt);
+    up(c->rt_hi, rthi.data(), sizeof(int) * nrt);
+    up(c->ct_lo, ctlo.data(), sizeof(int) * nct);
+    up(c->ct_hi, cthi.data(), sizeof(int) * nct);
+    up(c->kd_lo, kdlo.data(), sizeof(int) * M);
+    up(c->kd_hi, kdhi.data(), sizeof(int) * M);
+    up(c->P, Ph.data(), sizeof(double) * c->DN);
+    for (int s = 0; s < 6; s++) HIPC(hipMemsetAsync(c->slot[s], 0, sizeof(double) * c->MN, c->s));
+    HIPC(hipMemsetAsync(c->O, 0, sizeof(double) * c->DN, c->s));
+    HIPC(hipMemsetAsync(c->ik, 0, c->MN, c->s));
+    HIPC(hipMemsetAsync(c->scal, 0, sizeof(double) * S_NSCAL, c->s));
+    k_bdiag<<<c->row_blocks(), BLOCK, 0, c->s>>>(c->M, c->flo, c->flen, c->foff, c->fval, c->kd_lo, c->kd_hi,
+                                                 gamma, c->diag_full, c->diag_red);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->s));  // host vectors die at return
+    fill_maxr(c, -1);
+  } catch (...) {
+    destroy_ctx(c);
+    throw;
+  }
+  return c;
+}
+
+void copy_slot(pqdts_ctx *c, int slot, double *host, const double *src, int layout, bool to_device) {
+  double *dev = slot_ptr(c, slot);
+  const int M = c->M, N = c->N;
+  require(layout == PQDTS_ROW_MAJOR || layout == PQDTS_COL_MAJOR, "invalid layout");
+  require(host != nullptr || src != nullptr, "null host pointer");
+  std::vector<double> tmp;
+  if (to_device) {
+    const double *h = src;
+    if (layout == PQDTS_COL_MAJOR) {
+      tmp.resize(c->MN);
+      for (int n = 0; n < N; n++)
+        for (int i = 0; i < M; i++) tmp[(size_t)i * N + n] = src[(size_t)n * M + i];
+      h = tmp.data();
+    }
+    HIPC(hipMemcpyAsync(dev, h, sizeof(double) * c->MN, hipMemcpyHostToDevice, c->s));
+    HIPC(hipStreamSynchronize(c->s));
+  } else {
+    double *h = host;
+    if (layout == PQDTS_COL_MAJOR) {
+      tmp.resize(c->MN);
+      h = tmp.data();
+    }
+    HIPC(hipMemcpyAsync(h, dev, sizeof(double) * c->MN, hipMemcpyDeviceToHost, c->s));
+    HIPC(hipStreamSynchronize(c->s));
+    if (layout == PQDTS_COL_MAJOR)
+      for (int n = 0; n < N; n++)
+        for (int i = 0; i < M; i++) host[(size_t)n * M + i] = tmp[(size_t)i * N + n];
+  }
+}
+
+pqdts_ctx *checked(pqdts_ctx *c) {
+  require(c != nullptr, "null context");
+  return c;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *pqdts_last_error(void) { return g_err.c_str(); }
+
+int pqdts_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int pqdts_create(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *F_row, const int32_t *F_col,
+                 const double *F_val, int64_t nP, const int32_t *P_row, const int32_t *P_col, const double *P_val,
+                 double gamma, pqdts_ctx **out) {
+  return guarded([&] {
+    require(out != nullptr, "null output pointer");
+    *out = nullptr;
+    *out = create_ctx(M, N, D, nF, F_row, F_col, F_val, nP, P_row, P_col, P_val, gamma);
+  });
+}
+
+int pqdts_destroy(pqdts_ctx *ctx) {
+  return guarded([&] { destroy_ctx(ctx); });
+}
+
+int pqdts_put(pqdts_ctx *ctx, int slot, const double *host, int layout) {
+  return guarded([&] { copy_slot(checked(ctx), slot, nullptr, host, layout, true); });
+}
+
+int pqdts_get(pqdts_ctx *ctx, int slot, double *host, int layout) {
+  return guarded([&] { copy_slot(checked(ctx), slot, host, nullptr, layout, false); });
+}
+
+int pqdts_get_O(pqdts_ctx *ctx, double *host) {
+  return guarded([&] {
+    checked(ctx);
+    require(host != nullptr, "null host pointer");
+    HIPC(hipMemcpyAsync(host, ctx->O, sizeof(double) * ctx->DN, hipMemcpyDeviceToHost, ctx->s));
+    HIPC(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int pqdts_set_maxr(pqdts_ctx *ctx, const int32_t *maxr) {
+  return guarded([&] {
+    checked(ctx);
+    require(maxr != nullptr, "null maxr");
+    bool valid = true;
+    for (int i = 0; i < ctx->M; i++) {
+      require(maxr[i] >= -1 && maxr[i] < ctx->N, "maxr out of range at row " + std::to_string(i));
+      valid = valid && maxr[i] >= 0;
+    }
+    HIPC(hipMemcpyAsync(ctx->maxr, maxr, sizeof(int) * ctx->M, hipMemcpyHostToDevice, ctx->s));
+    HIPC(hipStreamSynchronize(ctx->s));
+    ctx->maxr_valid = valid;
+  });
+}
+
+int pqdts_get_maxr(pqdts_ctx *ctx, int32_t *maxr) {
+  return guarded([&] {
+    checked(ctx);
+    HIPC(hipMemcpyAsync(maxr, ctx->maxr, sizeof(int) * ctx->M, hipMemcpyDeviceToHost, ctx->s));
+    HIPC(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int pqdts_get_ik(pqdts_ctx *ctx, int8_t *ik) {
+  return guarded([&] {
+    checked(ctx);
+    HIPC(hipMemcpyAsync(ik, ctx->ik, ctx->MN, hipMemcpyDeviceToHost, ctx->s));
+    HIPC(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int pqdts_set_fullspace(pqdts_ctx *ctx, int full) {
+  return guarded([&] { checked(ctx)->full = full != 0; });
+}
+
+int pqdts_get_fullspace(pqdts_ctx *ctx, int *full) {
+  return guarded([&] { *full = checked(ctx)->full ? 1 : 0; });
+}
+
+int pqdts_L(pqdts_ctx *ctx, int x, double *val) {
+  return guarded([&] { *val = op_L(checked(ctx), slot_ptr(ctx, x)); });
+}
+
+int pqdts_dL(pqdts_ctx *ctx, int x, int g) {
+  return guarded([&] {
+    op_dL(checked(ctx), slot_ptr(ctx, x), slot_ptr(ctx, g));
+    HIPC(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int pqdts_BF(pqdts_ctx *ctx, int dd, int bx) {
+  return guarded([&] {
+    require(dd != bx, "BF input and output must differ");
+    op_BF(checked(ctx), slot_ptr(ctx, dd), slot_ptr(ctx, bx), false, false, false);
+    HIPC(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int pqdts_Bdiag(pqdts_ctx *ctx, double *diag) {
+  return guarded([&] {
+    checked(ctx);
+    if (diag) {
+      HIPC(hipMemcpyAsync(diag, ctx->diag(), sizeof(double) * ctx->M, hipMemcpyDeviceToHost, ctx->s));
+      HIPC(hipStreamSynchronize(ctx->s));
+    }
+  });
+}
+
+int pqdts_doproj(pqdts_ctx *ctx, int x) {
+  return guarded([&] {
+    op_doproj(checked(ctx), slot_ptr(ctx, x));
+    HIPC(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int pqdts_doproj2(pqdts_ctx *ctx, int x, double *xmin) {
+  return guarded([&] { *xmin = op_doproj2(checked(ctx), slot_ptr(ctx, x)); });
+}
+
+int pqdts_conv_test(pqdts_ctx *ctx, int x, int g, double *conv) {
+  return guarded([&] { *conv = op_conv(checked(ctx), slot_ptr(ctx, x), slot_ptr(ctx, g), false); });
+}
+
+int pqdts_pdot(pqdts_ctx *ctx, int x, int y, double *val) {
+  return guarded([&] { *val = op_pdot(checked(ctx), slot_ptr(ctx, x), slot_ptr(ctx, y)); });
+}
+
+#define BLAS1_BODY(z, x, y, a, b, kind)                  \
+  return guarded([&] {                                   \
+    op_axpby(checked(ctx), z, x, y, a, b, kind);         \
+    HIPC(hipStreamSynchronize(ctx->s));                  \
+  })
+
+int pqdts_paxpy(pqdts_ctx *ctx, double alpha, int x, int y) {
+  BLAS1_BODY(slot_ptr(ctx, y), slot_ptr(ctx, x), slot_ptr(ctx, y), alpha, 0.0, 1);
+}
+int pqdts_pgaxpy(pqdts_ctx *ctx, double alpha, int x, double beta, int y) {
+  BLAS1_BODY(slot_ptr(ctx, y), slot_ptr(ctx, x), slot_ptr(ctx, y), alpha, beta, 0);
+}
+int pqdts_pgaxpy2(pqdts_ctx *ctx, double alpha, int x, double beta, int y, int z) {
+  BLAS1_BODY(slot_ptr(ctx, z), slot_ptr(ctx, x), slot_ptr(ctx, y), alpha, beta, 0);
+}
+int pqdts_pset(pqdts_ctx *ctx, double alpha, int x) {
+  BLAS1_BODY(slot_ptr(ctx, x), nullptr, nullptr, alpha, 0.0, 4);
+}
+int pqdts_pcopy(pqdts_ctx *ctx, int x, int y) {
+  BLAS1_BODY(slot_ptr(ctx, y), slot_ptr(ctx, x), nullptr, 1.0, 0.0, 3);
+}
+int pqdts_pscale(pqdts_ctx *ctx, double alpha, int x) {
+  BLAS1_BODY(slot_ptr(ctx, x), slot_ptr(ctx, x), nullptr, alpha, 0.0, 2);
+}
+int pqdts_pcopy_scale(pqdts_ctx *ctx, double alpha, int x, int y) {
+  BLAS1_BODY(slot_ptr(ctx, y), slot_ptr(ctx, x), nullptr, alpha, 0.0, 2);
+}
+
+int pqdts_transform(pqdts_ctx *ctx) {
+  return guarded([&] {
+    op_transform(checked(ctx), true);
+    HIPC(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int pqdts_lalpha_min2(pqdts_ctx *ctx, double *alpha, double *dlalpha) {
+  return guarded([&] { op_linesearch(checked(ctx), false, alpha, dlalpha); });
+}
+
+int pqdts_lalpha_min_auglag(pqdts_ctx *ctx, double *alpha, double *dlalpha) {
+  return guarded([&] { op_linesearch(checked(ctx), true, alpha, dlalpha); });
+}
+
+int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage, int output, pqdts_event_cb cb,
+                void *user, double *conv) {
+  return guarded([&] {
+    checked(ctx);
+    require(mode == 1 || mode == 2, "mode must be 1 or 2");
+    require(start_stage == 1 || start_stage == 2, "start_stage must be 1 or 2");
+    ctx->cb = cb;
+    ctx->user = user;
+    try {
+      double cv = solve(ctx, mode, maxiter, eps, start_stage, output);
+      if (conv) *conv = cv;
+    } catch (...) {
+      ctx->cb = nullptr;
+      ctx->user = nullptr;
+      throw;
+    }
+    ctx->cb = nullptr;
+    ctx->user = nullptr;
+  });
+}
+
+int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *seconds) {
+  int n = 0;
+  int rc = guarded([&] {
+    checked(ctx);
+    for (const auto &name : ctx->timer_order) {
+      if (n < cap) {
+        const Timer &t = ctx->timers[name];
+        if (names) {
+          std::strncpy(names[n], name.c_str(), 23);
+          names[n][23] = 0;
+        }
+        if (counts) counts[n] = t.count;
+        if (seconds) seconds[n] = t.sec;
+      }
+      n++;
+    }
+  });
+  return rc == 0 ? n : -1;
+}
+
+int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n) {
+  return guarded([&] {
+    require(rows >= 0 && n >= 1 && rows * n < (int64_t)INT32_MAX * 8, "bad shape");
+    require(y != nullptr && x != nullptr && x != y, "x must be a distinct non-null buffer");
+    if (rows == 0) return;
+    double *d = dalloc<double>(rows * n);
+    hipError_t e1 = hipMemcpy(d, y, sizeof(double) * rows * n, hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) {
+      k_simplex_rows<<<cdiv(rows, WAVES_PER_BLOCK), BLOCK>>>(d, nullptr, nullptr, 0.0, (int)rows, (int)n, 0);
+      e1 = hipGetLastError();
+    }
+    if (e1 == hipSuccess) e1 = hipMemcpy(x, d, sizeof(double) * rows * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIPC(e1);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+"""Writes the golden fixtures under tests/golden/ from the reference build.
+
+Runs only where the upstream sources are present (they are compiled into
+oracle/_ref by oracle/Makefile.ref); the fixtures it writes are data and are
+committed. Nothing at test time runs the reference.
+
+Fixtures
+  cfg1_inputs.json        sha256 of the config-1 F and P arrays (generator drift guard)
+  cfg1_ref_prefix.json    reference stdout events of the config-1 solve (eps 1e-6, 1 thread)
+                          up to the end of the first stage-2 Newton iteration: the part of the
+                          trajectory that the reference reproduces across thread counts
+  cfg1_ref_states.npz     X after stage 1 (eps 1e-6, 1 thread) and the final X at eps 1e-11 with
+                          1 and 4 threads, with their objective values
+  condat_vectors.npz      inputs and outputs of the reference's condat_simplexproj.c
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+import oracle  # noqa: E402
+import reftrace  # noqa: E402
+from pqdts_amd import fortio, workloads  # noqa: E402
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def run_ref(prob, workdir, eps, maxiter, threads, output=2):
+    NF, NP = workloads.write_inputs(workdir, prob)
+    cmd = [oracle.REF_BIN, str(prob.M), str(prob.N), str(prob.D), str(NF), str(NP), "2", str(maxiter),
+           str(output), str(prob.gamma), str(eps), "1", "0", "0", "0"]
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), FORT_FMT_RECL="100000")
+    r = subprocess.run(cmd, cwd=workdir, env=env, stdout=subprocess.PIPE, text=True, check=True)
+    return r.stdout
+
+
+def prefix_events(ev):
+    """Events up to and including the first stage-2 'iter' event."""
+    out = []
+    for k, v in ev:
+        out.append([k, list(v)])
+        if k == reftrace.EV_ITER and int(v[0]) == 2:
+            break
+    return out
+
+
+def condat_cases(rng):
+    cases = [
+        np.array([0.5]), np.array([-3.0]), np.array([2.0, 2.0]), np.array([1.0, 0.0, 0.0]),
+        np.array([0.2, 0.3, 0.5]), np.full(7, 1.0 / 7), np.array([5.0, -1.0, 3.0, 3.0, 3.0]),
+        np.full(16, -2.0), np.array([1e-300, 1e300, -1e300]) * 0 + np.array([0.0, 1e3, -1e3]),
+    ]
+    for n in (2, 3, 50, 63, 64, 65, 1000, 4097):
+        cases.append(rng.normal(size=n))
+        cases.append(rng.random(n) / n * 2.0)
+        cases.append(np.round(rng.normal(size=n), 1))  # many ties
+    return cases
+
+
+def main():
+    oracle.build_ref()
+    prob = workloads.config1()
+    inputs = {"F_data": sha(prob.F.data), "F_indices": sha(prob.F.indices.astype(np.int64)),
+              "F_indptr": sha(prob.F.indptr.astype(np.int64)), "P": sha(prob.P),
+              "nnzF": int(prob.F.nnz), "nnzP": int(np.count_nonzero(prob.P))}
+    with open(os.path.join(HERE, "cfg1_inputs.json"), "w") as f:
+        json.dump(inputs, f, indent=1)
+
+    with tempfile.TemporaryDirectory() as w:
+        out = run_ref(prob, w, 1e-6, 200, 1)
+        ev = reftrace.parse(out)
+        with open(os.path.join(HERE, "cfg1_ref_prefix.json"), "w") as f:
+            json.dump({"cmd_eps": 1e-6, "threads": 1, "events": prefix_events(ev)}, f)
+        _, _, _, _, _, X1 = fortio.read_state(os.path.join(w, "rank_     0_oiter     1.dat"))
+    states = {"stage1_eps1e-6": X1}
+    for th in (1, 4):
+        with tempfile.TemporaryDirectory() as w:
+            out = run_ref(prob, w, 1e-11, 2000, th)
+            ev = reftrace.parse(out)
+            its = [v for k, v in ev if k == reftrace.EV_ITER]
+            _, _, _, _, _, Xf = fortio.read_state(os.path.join(w, "rank_     0_final.dat"))
+            states[f"final_eps1e-11_t{th}"] = Xf
+            states[f"final_eps1e-11_t{th}_L"] = np.array(its[-1][2])
+            states[f"final_eps1e-11_t{th}_iters"] = np.array([len(its)])
+    np.savez_compressed(os.path.join(HERE, "cfg1_ref_states.npz"), **states)
+
+    rng = np.random.default_rng(7)
+    ys, xs = [], []
+    for y in condat_cases(rng):
+        ys.append(y)
+        xs.append(oracle.ref_condat(y))
+    np.savez_compressed(os.path.join(HERE, "condat_vectors.npz"),
+                        **{f"y{i}": y for i, y in enumerate(ys)}, **{f"x{i}": x for i, x in enumerate(xs)})
+    print("wrote fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    main()

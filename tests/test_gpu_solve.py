@@ -1,0 +1,111 @@
+"""End-to-end parity of the GPU FP64 solve (pqdts_solve through the C ABI).
+
+1. Trajectory: over the part of the config-1 solve that the reference
+   reproduces across its own thread counts (stage 1 and the first stage-2
+   Newton iteration), the GPU solve reports the same events as the reference
+   binary's stdout (golden fixture): identical PCG/line-search counts, alphas
+   and active-set sizes, objective to 1e-12.
+2. Endpoint: at eps = 1e-11 the GPU solution is within the reference's own
+   1-vs-4-thread spread (and within 1e-6) of the reference's final Pi, with the
+   same active set (zero pattern), and the objective agrees to 1e-9.
+3. Invariants of every returned Pi: rows on the simplex.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pqdts_amd
+import reftrace
+from pqdts_amd import workloads
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def test_trajectory_prefix_matches_reference(cfg1):
+    ref = json.load(open(os.path.join(GOLD, "cfg1_ref_prefix.json")))["events"]
+    ctx = pqdts_amd.Context.from_problem(cfg1)
+    ctx.put("X", np.full((cfg1.M, cfg1.N), 1.0 / cfg1.N))
+    n_ref = len(ref)
+    seen = []
+
+    def stop_after_prefix(c, kind, v):
+        seen.append((kind, v))
+
+    _, ev = ctx.solve(2, 200, 1e-6, 1, 0, on_event=stop_after_prefix)
+    ev = [e for e in ev if e[0] not in (pqdts_amd._lib.EV_STAGE,)][:n_ref]
+    assert [k for k, _ in ev] == [k for k, _ in ref]
+    for (k, a), (_, b) in zip(ev, ref):
+        a, b = np.array(a), np.array(b)
+        if k == reftrace.EV_ACTIVE:
+            assert a[0] == b[0]
+        elif k == reftrace.EV_PCG:
+            assert a[0] == b[0] and _rel(a[1:], b[1:]) <= 1e-9
+        elif k == reftrace.EV_LS:
+            assert a[0] == b[0] and a[2] == b[2] and _rel(a[1], b[1]) <= 1e-12
+        elif k == reftrace.EV_ITER:
+            assert tuple(a[:2]) == tuple(b[:2]) and a[7] == b[7]
+            assert _rel(a[2], b[2]) <= 1e-12 and _rel(a[4], b[4]) <= 1e-9
+
+
+def test_tight_endpoint_matches_reference(cfg1):
+    z = np.load(os.path.join(GOLD, "cfg1_ref_states.npz"))
+    X1, X4 = z["final_eps1e-11_t1"], z["final_eps1e-11_t4"]
+    spread = float(np.max(np.abs(X1 - X4)))
+    ctx = pqdts_amd.Context.from_problem(cfg1)
+    ctx.put("X", np.full((cfg1.M, cfg1.N), 1.0 / cfg1.N))
+    conv, ev = ctx.solve(2, 2000, 1e-11, 1, 0)
+    X = ctx.get("X")
+    assert conv < 1e-11
+    err = float(np.max(np.abs(X - X1)))
+    print(f"||dPi||inf vs reference = {err:.3e}; reference 1-vs-4-thread spread = {spread:.3e}")
+    assert err <= max(1e-6, 20 * spread)
+    its = [v for k, v in ev if k == reftrace.EV_ITER]
+    assert _rel(its[-1][2], float(z["final_eps1e-11_t1_L"])) <= 1e-9
+    # same active set: entries that are exactly zero in both, up to the tolerance
+    assert np.array_equal(X1 > 1e-9, X > 1e-9)
+    assert np.allclose(X.sum(1), 1.0, atol=1e-12) and X.min() >= 0.0
+
+
+def test_mode1_projected_gradient_runs_and_decreases(cfg1):
+    ctx = pqdts_amd.Context.from_problem(cfg1)
+    ctx.put("X", np.full((cfg1.M, cfg1.N), 1.0 / cfg1.N))
+    conv, ev = ctx.solve(1, 5, 1e-30, 1, 0)
+    its = [v for k, v in ev if k == reftrace.EV_ITER]
+    assert len(its) == 10  # mode 1 runs maxiter in both stages
+    Ls = [v[2] for v in its]
+    assert all(b <= a + 1e-15 for a, b in zip(Ls, Ls[1:]))
+    X = ctx.get("X")
+    assert np.allclose(X.sum(1), 1.0, atol=1e-12) and X.min() >= -1e-12
+
+
+def test_mode1_matches_oracle(cfg1):
+    import oracle
+    ctx = pqdts_amd.Context.from_problem(cfg1)
+    orc = oracle.Oracle.from_problem(cfg1)
+    ctx.put("X", np.full((cfg1.M, cfg1.N), 1.0 / cfg1.N))
+    orc.buf("X")[:] = 1.0 / cfg1.N
+    _, ev = ctx.solve(1, 3, 1e-30, 1, 0)
+    _, evo = orc.solve(1, 3, 1e-30, 1, 0)
+    a = [v for k, v in ev if k == reftrace.EV_ITER]
+    b = [v for k, v in evo if k == reftrace.EV_ITER]
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert x[7] == y[7] and _rel(x[2], y[2]) <= 1e-10
+
+
+def test_start_stage2_and_maxiter_cap(cfg1):
+    ctx = pqdts_amd.Context.from_problem(cfg1)
+    ctx.put("X", np.full((cfg1.M, cfg1.N), 1.0 / cfg1.N))
+    conv, ev = ctx.solve(2, 4, 1e-30, 2, 0)
+    its = [v for k, v in ev if k == reftrace.EV_ITER]
+    assert [int(v[0]) for v in its] == [2, 2, 2, 2]
+    stages = [v for k, v in ev if k == pqdts_amd._lib.EV_STAGE]
+    assert [int(v[0]) for v in stages] == [2]
