@@ -53,7 +53,7 @@ enum pqdts_layout { PQDTS_ROW_MAJOR = 0, PQDTS_COL_MAJOR = 1 };
 enum pqdts_event {
   PQDTS_EV_ITER = 0,    /* oiter iiter L xmin conv ||der||^2 mode alpha dlalpha  (pqdts.f90:2131) */
   PQDTS_EV_PCG = 1,     /* j sqrt(b2) eps b1 v2                                (pqdts.f90:2084) */
-  PQDTS_EV_LS = 2,      /* i L alpha df xmin                                   (pqdts.f90:429/474) */
+  PQDTS_EV_LS = 2,      /* i L alpha df xmin is_min2 (1: Lalpha_min2, 0: auglag) (pqdts.f90:429/474) */
   PQDTS_EV_ACTIVE = 3,  /* nact M*N                                            (pqdts.f90:2002) */
   PQDTS_EV_STOP1 = 4,   /* dlalpha                                             (pqdts.f90:1956) */
   PQDTS_EV_ITER0 = 5,   /* L0 sqrt(||der||^2)                                  (pqdts.f90:1914) */
@@ -141,6 +141,19 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
  * reference's timers: "L", "dL", "BF", "Lalpha_min", ...). Returns the number of
  * timers; fills up to `cap` entries. */
 int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *seconds);
+
+/* Per-kernel device timing with HIP events on the context's stream: `on`
+ * enables it and clears the totals; pqdts_kernel_stats returns the number of
+ * kernel names ("spmm_F" = O = F*V, "spmmT_F" = 2 F^T W + stencil) and fills
+ * launch counts and total milliseconds. No counterpart in the reference (its
+ * timers are the wall-clock ones above, pqdts.f90:103-186). */
+int pqdts_kernel_timing(pqdts_ctx *ctx, int on);
+int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms);
+
+/* Sizes of a context: info[0..7] = M, N, D, stored F elements (sum of row span
+ * lengths, the reference's Frows storage), elements swept by the F*V tiles,
+ * #row tiles, #column tiles, 0. */
+int pqdts_info(pqdts_ctx *ctx, int64_t *info);
 
 /* simplexproj_condat_ (condat_simplexproj.c:60) over many rows at once: rows of
  * length n from y (host, row-major) projected onto the simplex into x (host).

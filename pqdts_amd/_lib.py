@@ -26,6 +26,7 @@ EXPORTS = [
     "pqdts_conv_test", "pqdts_pdot", "pqdts_paxpy", "pqdts_pgaxpy", "pqdts_pgaxpy2", "pqdts_pset",
     "pqdts_pcopy", "pqdts_pscale", "pqdts_pcopy_scale", "pqdts_transform", "pqdts_lalpha_min2",
     "pqdts_lalpha_min_auglag", "pqdts_solve", "pqdts_timers", "pqdts_simplexproj_rows",
+    "pqdts_kernel_timing", "pqdts_kernel_stats", "pqdts_info",
 ]
 
 _lib = None
@@ -77,6 +78,9 @@ def load():
     L.pqdts_solve.argtypes = [vp, i32, i32, d, i32, i32, EVENT_CB, vp, dp]
     L.pqdts_timers.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
     L.pqdts_simplexproj_rows.argtypes = [dp, dp, i64, i64]
+    L.pqdts_kernel_timing.argtypes = [vp, i32]
+    L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
+    L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
     _lib = L
     return L
 
@@ -264,6 +268,27 @@ class Context:
         _check(load().pqdts_solve(self.h, int(mode), int(maxiter), float(eps), int(start_stage), int(output), fn,
                                   None, C.byref(conv)))
         return conv.value, events
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 8)()
+        _check(load().pqdts_info(self.h, a))
+        return {"M": a[0], "N": a[1], "D": a[2], "nspan": a[3], "tile_elems": a[4], "row_tiles": a[5],
+                "col_tiles": a[6]}
+
+    def kernel_timing(self, on: bool):
+        _check(load().pqdts_kernel_timing(self.h, int(bool(on))))
+
+    def kernel_stats(self) -> dict:
+        """{kernel: (launches, total_ms)} from HIP events on the context's stream."""
+        L = load()
+        n = L.pqdts_kernel_stats(self.h, 0, None, None, None)
+        if n <= 0:
+            return {}
+        names = (C.c_char * 24 * n)()
+        counts = (C.c_int * n)()
+        ms = (C.c_double * n)()
+        L.pqdts_kernel_stats(self.h, n, C.cast(names, C.c_void_p), counts, ms)
+        return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], ms[i]) for i in range(n)}
 
     def timers(self):
         L = load()

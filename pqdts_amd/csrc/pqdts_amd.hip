@@ -78,6 +78,8 @@ struct Timer {
 struct pqdts_ctx {
   int M = 0, N = 0, D = 0;
   int64_t MN = 0, DN = 0;
+  int64_t nspan = 0;       // stored F elements: sum of row span lengths
+  int64_t tile_elems = 0;  // elements swept by the two products (union ranges x tile heights)
   double gamma = 0.0;
   int device = 0;
   hipStream_t s = nullptr;
@@ -106,6 +108,62 @@ struct pqdts_ctx {
   std::vector<std::string> timer_order;
   pqdts_event_cb cb = nullptr;
   void *user = nullptr;
+  // per-kernel HIP-event timing (pqdts_kernel_timing): pending start/stop pairs
+  // on the context's stream, folded into totals at every host synchronisation
+  bool ktiming = false;
+  struct KPair {
+    hipEvent_t a, b;
+    int kid;
+  };
+  std::vector<KPair> kpending;
+  std::vector<hipEvent_t> kpool;
+  std::vector<std::string> knames;
+  std::vector<int> kcount;
+  std::vector<double> kms;
+
+  int kernel_id(const char *name) {
+    for (size_t i = 0; i < knames.size(); i++)
+      if (knames[i] == name) return (int)i;
+    knames.push_back(name);
+    kcount.push_back(0);
+    kms.push_back(0.0);
+    return (int)knames.size() - 1;
+  }
+  hipEvent_t kevent() {
+    if (!kpool.empty()) {
+      hipEvent_t e = kpool.back();
+      kpool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPC(hipEventCreate(&e));
+    return e;
+  }
+  // returns the start event recorded before a launch (or nullptr)
+  hipEvent_t kbegin() {
+    if (!ktiming) return nullptr;
+    hipEvent_t a = kevent();
+    HIPC(hipEventRecord(a, s));
+    return a;
+  }
+  void kend(hipEvent_t a, const char *name) {
+    if (!a) return;
+    hipEvent_t b = kevent();
+    HIPC(hipEventRecord(b, s));
+    kpending.push_back({a, b, kernel_id(name)});
+  }
+  // called after the stream has been synchronised
+  void kfold() {
+    for (auto &p : kpending) {
+      float ms = 0.f;
+      HIPC(hipEventElapsedTime(&ms, p.a, p.b));
+      kcount[p.kid]++;
+      kms[p.kid] += ms;
+      kpool.push_back(p.a);
+      kpool.push_back(p.b);
+    }
+    kpending.clear();
+  }
 
   double *reg(int r) { return part + (int64_t)r * pcap; }
   const double *diag() const { return full ? diag_full : diag_red; }
@@ -137,6 +195,7 @@ struct pqdts_ctx {
   void fetch() {
     HIPC(hipMemcpyAsync(hscal, scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
+    if (!kpending.empty()) kfold();
   }
   void reduce(int region, int n, int dst, bool is_min = false) {
     k_reduce<<<1, BLOCK, 0, s>>>(reg(region), n, scal, dst, is_min ? 1 : 0);
@@ -182,6 +241,7 @@ void op_reconstruct(pqdts_ctx *c, double *X) {
 int op_spmm(pqdts_ctx *c, const double *V, bool expand, bool loss, bool store) {
   dim3 grid(cdiv(c->N, BLOCK), c->nrt);
   double *part = c->reg(R_A);
+  hipEvent_t ev0 = c->kbegin();
 #define SPMM_ARGS                                                                                      \
   V, c->N, c->D, c->flo, c->flen, c->foff, c->fval, c->fhas, c->rt_lo, c->rt_hi, c->maxr, c->ev, c->P, \
       c->O, part
@@ -195,6 +255,7 @@ int op_spmm(pqdts_ctx *c, const double *V, bool expand, bool loss, bool store) {
     k_spmm_rows<RD, KC, false, false, true><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
 #undef SPMM_ARGS
   HIPC(hipGetLastError());
+  c->kend(ev0, "spmm_F");
   return grid.x * grid.y;
 }
 
@@ -214,6 +275,7 @@ int op_spmmT(pqdts_ctx *c, int mode, const double *E, const double *dd, double *
   }
   dim3 grid(cdiv(c->N, BLOCK), c->nct);
   double *part = c->reg(R_B);
+  hipEvent_t ev0 = c->kbegin();
 #define SPMMT_ARGS                                                                                        \
   c->O, c->P, c->N, c->M, c->D, c->flo, c->flen, c->foff, c->fval, c->ct_lo, c->ct_hi, c->gamma, E, c->maxr, \
       c->ev, c->gm, c->ik, dd, G, part
@@ -237,6 +299,7 @@ int op_spmmT(pqdts_ctx *c, int mode, const double *E, const double *dd, double *
   }
 #undef SPMMT_ARGS
   HIPC(hipGetLastError());
+  c->kend(ev0, "spmmT_F");
   return grid.x * grid.y;
 }
 
@@ -370,7 +433,7 @@ void op_linesearch(pqdts_ctx *c, bool condat, double *alpha_out, double *dlalpha
       lm = Lp;
       alphamin = alpha;
     }
-    c->emit(PQDTS_EV_LS, {(double)i, Lp, alpha, df, xmin});
+    c->emit(PQDTS_EV_LS, {(double)i, Lp, alpha, df, xmin, condat ? 0.0 : 1.0});
     if (Lp > L0 + 0.1 * alpha * df || !feasible)
       alpha = alpha * 0.75;
     else
@@ -496,11 +559,18 @@ double solve(pqdts_ctx *c, int mode, int maxiter, double conv_thres, int start_s
     c->emit(PQDTS_EV_STAGE, {(double)oiter, conv});
   }
   HIPC(hipStreamSynchronize(c->s));
+  c->kfold();
   return conv;
 }
 
 void destroy_ctx(pqdts_ctx *c) {
   if (!c) return;
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  for (auto &p : c->kpending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (hipEvent_t e : c->kpool) (void)hipEventDestroy(e);
   void *ptrs[] = {c->flo,  c->flen, c->foff, c->fval,      c->fhas,     c->rt_lo, c->rt_hi, c->ct_lo,
                   c->ct_hi, c->kd_lo, c->kd_hi, c->P,        c->O,        c->diag_full, c->diag_red, c->maxr,
                   c->ik,   c->ev,   c->gm,   c->lam,       c->lmin,     c->part,  c->scal};
@@ -600,6 +670,8 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     HIPC(hipGetDevice(&c->device));
     HIPC(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     c->nrt = nrt;
+    c->nspan = off[D];
+    for (int t = 0; t < nrt; t++) c->tile_elems += (int64_t)RD * std::max(0, rthi[t] - rtlo[t] + 1);
     c->nct = nct;
     c->flo = dalloc<int>(D);
     c->flen = dalloc<int>(D);
@@ -909,6 +981,53 @@ int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double
     }
   });
   return rc == 0 ? n : -1;
+}
+
+int pqdts_kernel_timing(pqdts_ctx *ctx, int on) {
+  return guarded([&] {
+    checked(ctx);
+    HIPC(hipStreamSynchronize(ctx->s));
+    ctx->kfold();
+    ctx->ktiming = on != 0;
+    for (size_t i = 0; i < ctx->kcount.size(); i++) {
+      ctx->kcount[i] = 0;
+      ctx->kms[i] = 0.0;
+    }
+  });
+}
+
+int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms) {
+  int n = 0;
+  int rc = guarded([&] {
+    checked(ctx);
+    HIPC(hipStreamSynchronize(ctx->s));
+    ctx->kfold();
+    n = (int)ctx->knames.size();
+    for (int i = 0; i < std::min(n, cap); i++) {
+      if (names) {
+        std::strncpy(names[i], ctx->knames[i].c_str(), 23);
+        names[i][23] = 0;
+      }
+      if (counts) counts[i] = ctx->kcount[i];
+      if (ms) ms[i] = ctx->kms[i];
+    }
+  });
+  return rc == 0 ? n : -1;
+}
+
+int pqdts_info(pqdts_ctx *ctx, int64_t *info) {
+  return guarded([&] {
+    checked(ctx);
+    require(info != nullptr, "null info");
+    info[0] = ctx->M;
+    info[1] = ctx->N;
+    info[2] = ctx->D;
+    info[3] = ctx->nspan;
+    info[4] = ctx->tile_elems;
+    info[5] = ctx->nrt;
+    info[6] = ctx->nct;
+    info[7] = 0;
+  });
 }
 
 int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n) {

@@ -1,0 +1,127 @@
+"""The Fortran drop-in (pqdts_amd/bin/pqdts_omp.x) driven the way pqdts.py
+drives the reference (pqdts.py:15-107): 14 positional arguments, COO inputs
+under data/, outputs parsed with pqdts.py's own rules:
+  * objective = float(line.split()[4]) of lines starting with " iter   "
+  * memory    = VmHWM from 'status_           0 .out'
+  * POVM      = 'rank_     0_final.dat' read with scipy FortranFile.read_ints(int32)
+                for the header records and read_reals(float) for X (column-major)
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.io import FortranFile
+
+import oracle
+from pqdts_amd import fortio, workloads
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "pqdts_amd", "bin", "pqdts_omp.x")
+
+
+def run(workdir, prob, NF, NP, maxiter=200, output=1, eps=1e-6, start_stage=1, read_input=0, smo=0, bench=0,
+        mode=2):
+    cmd = [BIN, str(prob.M), str(prob.N), str(prob.D), str(NF), str(NP), str(mode), str(maxiter), str(output),
+           str(prob.gamma), str(eps), str(start_stage), str(read_input), str(smo), str(bench)]
+    r = subprocess.run(cmd, cwd=workdir, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    return r.stdout
+
+
+def pqdts_py_objective(stdout):
+    O = None
+    for line in stdout.splitlines(True):
+        if line.startswith(" iter   "):
+            O = float(line.split()[4])
+    return O
+
+
+def pqdts_py_final(workdir):
+    fi = FortranFile(os.path.join(workdir, "rank_     0_final.dat"), "r")
+    M = fi.read_ints(np.int32)[0]
+    N = fi.read_ints(np.int32)[0]
+    D = fi.read_ints(np.int32)[0]
+    ML = fi.read_ints(np.int32)[0]
+    rank = fi.read_ints(np.int32)[0]
+    X = fi.read_reals(float).reshape((ML, N), order="F")
+    return M, N, D, ML, rank, X
+
+
+def test_driver_cfg1_pqdts_py_contract(tmp_path, cfg1):
+    NF, NP = workloads.write_inputs(str(tmp_path), cfg1)
+    out = run(str(tmp_path), cfg1, NF, NP, maxiter=200, output=2)
+    L = pqdts_py_objective(out)
+    assert L is not None and 1e-4 < L < 1e-2
+    M, N, D, ML, rank, X = pqdts_py_final(str(tmp_path))
+    assert (M, N, D, ML, rank) == (cfg1.M, cfg1.N, cfg1.D, cfg1.M, 0)
+    assert np.allclose(X.sum(1), 1.0, atol=1e-12) and X.min() >= 0
+    # per-stage files in the reference's record layout
+    for k in (1, 2):
+        p = tmp_path / f"rank_     0_oiter     {k}.dat"
+        assert p.exists()
+        assert p.stat().st_size == 4 * 16 + 12 + 8 + 8 * cfg1.M * cfg1.N
+    Mh, Nh, Dh, lr, rk, X2 = fortio.read_state(str(tmp_path / "rank_     0_oiter     2.dat"))
+    assert np.array_equal(X2, X)
+    st = open(tmp_path / "status_           0 .out").read().splitlines()
+    hwm = [float(o.split()[1]) for o in st if o.startswith(" VmHWM:")]
+    assert hwm and hwm[0] > 0
+    timing = open(tmp_path / "timing_           0 .out").read().splitlines()
+    rows = [t.split() for t in timing]
+    assert {r[0] for r in rows} >= {"0", "1", "2", "-1"}
+    assert any(r[1] == "BF" for r in rows) and any(r[1] == "min" for r in rows)
+    # the objective the wrapper sees is the solver's final objective
+    iters = [ln for ln in out.splitlines() if ln.startswith(" iter   ")]
+    assert len(iters) > 20
+
+
+def test_driver_restart_from_initial_file(tmp_path, cfg1):
+    """read_input=1 with the file pqdts.py writes (8-byte rank record, pqdts.py:40-58)."""
+    NF, NP = workloads.write_inputs(str(tmp_path), cfg1)
+    X0 = cfg1.pi_true * 0.5 + 0.5 / cfg1.N
+    fortio.write_state(str(tmp_path / "data" / "initial_     0.dat"), cfg1.M, cfg1.N, cfg1.D, X0, rank_bytes=8)
+    out = run(str(tmp_path), cfg1, NF, NP, maxiter=0, output=1, read_input=1)
+    assert "reading starting state from" in out
+    # maxiter = 0: no Newton step, the state is written back unchanged
+    _, _, _, _, _, X = pqdts_py_final(str(tmp_path))
+    assert np.max(np.abs(X - X0)) == 0.0
+    # the iteration-0 objective equals L(X0)
+    o = oracle.Oracle.from_problem(cfg1)
+    o.buf("X")[:] = X0
+    L0 = o.L("X")
+    line = [ln for ln in out.splitlines() if ln.startswith(" iter   ")][0]
+    assert abs(float(line.split()[4]) - L0) <= 1e-12 * L0
+
+
+def test_driver_header_mismatch_stops(tmp_path, cfg1):
+    NF, NP = workloads.write_inputs(str(tmp_path), cfg1)
+    fortio.write_state(str(tmp_path / "data" / "initial_     0.dat"), cfg1.M + 1, cfg1.N, cfg1.D,
+                       np.zeros((cfg1.M + 1, cfg1.N)))
+    out = run(str(tmp_path), cfg1, NF, NP, maxiter=5, output=1, read_input=1)
+    assert "different M" in out
+    assert not (tmp_path / "rank_     0_final.dat").exists()
+
+
+def test_driver_bench_mode(tmp_path, cfg1):
+    NF, NP = workloads.write_inputs(str(tmp_path), cfg1)
+    out = run(str(tmp_path), cfg1, NF, NP, maxiter=1, output=0, bench=1)
+    for op in ("L", "dL", "pset", "pcopy", "pscale", "pcopy_scale", "paxpy", "pgaxpy", "pgaxpy2", "pdot",
+               "doproj", "Bdiag", "BF"):
+        assert sum(1 for ln in out.splitlines() if ln.split()[:2] == ["t", op]) == 10, op
+
+
+def test_driver_internal_F(tmp_path):
+    """Argument 4 = -1: F generated inside the program (M = (D-1)^2)."""
+    D = 12
+    M, N = (D - 1) ** 2, 6
+    rng = np.random.default_rng(0)
+    P = rng.random((D, N))
+    P /= P.sum(1, keepdims=True)
+    prob = workloads.Problem("intF", M, N, D, None, P, 0.0, 1e-6)
+    import scipy.sparse as sps
+    NP = workloads.write_coo(str(tmp_path / "data"), "P", sps.csr_matrix(P))
+    out = run(str(tmp_path), prob, -1, NP, maxiter=3, output=1)
+    assert "Fsize" in out
+    _, _, _, _, _, X = pqdts_py_final(str(tmp_path))
+    assert np.allclose(X.sum(1), 1.0, atol=1e-12)
