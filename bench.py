@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--maxiter", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--kernels", type=int, default=0,
+                    help="instead of solving: run each hot operator this many times on the same problem "
+                         "(plus a plain 16 MB copy for counter calibration) and exit; for profiler passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems shard "
@@ -118,6 +121,21 @@ def main():
     ctx = pqdts_amd.Context.from_problem(prob)
     info = ctx.info()
     x0 = 1.0 / prob.N
+
+    if args.kernels:
+        rng = np.random.default_rng(0)
+        X = rng.random((prob.M, prob.N))
+        ctx.put("X", X / X.sum(1, keepdims=True))
+        ctx.put("XTMP", rng.normal(size=(prob.M, prob.N)))
+        for r in range(args.kernels):
+            for s in ("G", "HP", "SN"):  # 48 MB of writes push X out of the 32 MB of L2
+                ctx.pset(0.0, s)
+            ctx.pcopy("X", "Z")          # 16 MB read + 16 MB write: counter calibration
+            ctx.L("X")
+            ctx.dL("X", "G")
+            ctx.BF("XTMP", "HP")
+        log(f"ran {args.kernels} x (pcopy, L, dL, BF) on cfg2")
+        return
 
     def progress(_c, kind, v):
         if kind == 0 and int(v[1]) % 25 == 0:

@@ -1,0 +1,53 @@
+"""Summaries of rocprofv3 rocpd databases (kernel trace and PMC passes).
+
+  python -m pqdts_amd.profsum trace <db>                 per-kernel calls / avg / total time
+  python -m pqdts_amd.profsum pmc <db> [<db> ...]         per-kernel mean counter value per dispatch
+                                                          (+ VGPRs, LDS bytes, grid)
+Kernel names are shortened to their template head.
+"""
+import sqlite3
+import sys
+from collections import defaultdict
+
+
+def short(name: str, keep_args: bool = False) -> str:
+    n = name.split("(")[0] if not keep_args else name
+    return n.replace("void ", "").replace("pq::", "")
+
+
+def trace(db: str):
+    c = sqlite3.connect(db)
+    rows = c.execute("select name, count(*), avg(duration), sum(duration), min(duration), max(duration) "
+                     "from kernels group by name order by sum(duration) desc").fetchall()
+    total = sum(r[3] for r in rows)
+    print(f"{'kernel':58s} {'calls':>7s} {'avg_us':>10s} {'total_ms':>10s} {'pct':>6s}")
+    for name, calls, avg, tot, mn, mx in rows:
+        print(f"{short(name)[:58]:58s} {calls:7d} {avg / 1e3:10.2f} {tot / 1e6:10.2f} {100 * tot / total:6.2f}")
+    print(f"{'total':58s} {'':7s} {'':10s} {total / 1e6:10.2f}")
+
+
+def pmc(dbs):
+    agg = defaultdict(lambda: defaultdict(list))
+    meta = {}
+    for db in dbs:
+        c = sqlite3.connect(db)
+        for kname, cname, val, vg, avg_, lds, gs, dur in c.execute(
+                "select kernel_name, counter_name, value, vgpr_count, accum_vgpr_count, lds_block_size, "
+                "grid_size, duration from counters_collection"):
+            agg[short(kname)][cname].append(val)
+            meta[short(kname)] = (vg, avg_, lds, gs)
+    counters = sorted({cn for k in agg for cn in agg[k]})
+    print(f"{'kernel':58s} {'dispatches':>10s} " + " ".join(f"{cn + '/disp':>16s}" for cn in counters)
+          + f" {'vgpr':>5s} {'agpr':>5s} {'lds_B':>7s} {'grid':>9s}")
+    for k in sorted(agg):
+        n = max(len(v) for v in agg[k].values())
+        vals = [sum(agg[k][cn]) / len(agg[k][cn]) if agg[k][cn] else float("nan") for cn in counters]
+        vg, ag, lds, gs = meta[k]
+        print(f"{k[:58]:58s} {n:10d} " + " ".join(f"{v:16.1f}" for v in vals) + f" {vg:5d} {ag:5d} {lds:7d} {gs:9d}")
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "trace":
+        trace(sys.argv[2])
+    else:
+        pmc(sys.argv[2:])
