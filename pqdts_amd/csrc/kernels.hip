@@ -99,102 +99,6 @@ __global__ __launch_bounds__(BLOCK) void k_reduce(const double *__restrict__ par
 }
 
 // ---------------------------------------------------------------------------
-// O = F V  (the reference's redO local part, pqdts.f90:724-736)
-//
-// Block = RD consecutive F rows x 256 consecutive columns n. The union k-range
-// of the RD row spans is swept in chunks of KC; each chunk of F is staged in
-// LDS (zero-filled outside a row's span) and every V element fetched from HBM/L2
-// is reused by RD fused multiply-adds from registers.
-//   EXPAND: the reduced-space BF input (pqdts.f90:260-267): V[k, maxr[k]] is
-//           replaced by ev[k] = -(sum_n V[k,n] - V[k,maxr[k]]).
-//   LOSS:   the objective's data term (pqdts.f90:944-952): per-block partial of
-//           sum (O - P)^2 over rows that have entries.
-//   STORE:  write O.
-// ---------------------------------------------------------------------------
-template <int RD, int KC, bool EXPAND, bool LOSS, bool STORE>
-__global__ __launch_bounds__(BLOCK) void k_spmm_rows(
-    const double *__restrict__ V, int N, int D, const int *__restrict__ flo, const int *__restrict__ flen,
-    const int64_t *__restrict__ foff, const double *__restrict__ fval, const uint8_t *__restrict__ fhas,
-    const int *__restrict__ tlo, const int *__restrict__ thi, const int *__restrict__ maxr,
-    const double *__restrict__ ev, const double *__restrict__ P, double *__restrict__ O,
-    double *__restrict__ partial) {
-  __shared__ double Fs[RD][KC];
-  __shared__ int s_lo[RD], s_len[RD];
-  __shared__ int64_t s_off[RD];
-  __shared__ double sh[WAVES_PER_BLOCK];
-  const int tile = blockIdx.y;
-  const int d0 = tile * RD;
-  const int n = blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = n < N;
-  if (threadIdx.x < RD) {
-    int d = d0 + threadIdx.x;
-    bool ok = d < D && fhas[d];
-    s_lo[threadIdx.x] = ok ? flo[d] : 0;
-    s_len[threadIdx.x] = ok ? flen[d] : 0;
-    s_off[threadIdx.x] = ok ? foff[d] : 0;
-  }
-  double acc[RD];
-#pragma unroll
-  for (int r = 0; r < RD; r++) acc[r] = 0.0;
-  const int ulo = tlo[tile], uhi = thi[tile];
-  __syncthreads();
-  for (int k0 = ulo; k0 <= uhi; k0 += KC) {
-    const int kc = min(KC, uhi - k0 + 1);
-    for (int e = threadIdx.x; e < RD * KC; e += BLOCK) {
-      const int r = e / KC, kk = e - r * KC;
-      const int k = k0 + kk;
-      double v = 0.0;
-      const int rel = k - s_lo[r];
-      if (kk < kc && rel >= 0 && rel < s_len[r]) v = fval[s_off[r] + rel];
-      Fs[r][kk] = v;
-    }
-    __syncthreads();
-    if (active) {
-      const double *vp = V + (int64_t)k0 * N + n;
-      int kk = 0;
-      for (; kk + 1 < kc; kk += 2) {
-        double v0 = vp[(int64_t)kk * N];
-        double v1 = vp[(int64_t)(kk + 1) * N];
-        if (EXPAND) {
-          if (n == maxr[k0 + kk]) v0 = ev[k0 + kk];
-          if (n == maxr[k0 + kk + 1]) v1 = ev[k0 + kk + 1];
-        }
-#pragma unroll
-        for (int r = 0; r < RD; r++) {
-          acc[r] = fma(Fs[r][kk], v0, acc[r]);
-          acc[r] = fma(Fs[r][kk + 1], v1, acc[r]);
-        }
-      }
-      if (kk < kc) {
-        double v0 = vp[(int64_t)kk * N];
-        if (EXPAND && n == maxr[k0 + kk]) v0 = ev[k0 + kk];
-#pragma unroll
-        for (int r = 0; r < RD; r++) acc[r] = fma(Fs[r][kk], v0, acc[r]);
-      }
-    }
-    __syncthreads();
-  }
-  double loss = 0.0;
-  if (active) {
-#pragma unroll
-    for (int r = 0; r < RD; r++) {
-      const int d = d0 + r;
-      if (d < D) {
-        if (STORE) O[(int64_t)d * N + n] = acc[r];
-        if (LOSS && s_len[r] > 0) {
-          double t = acc[r] - P[(int64_t)d * N + n];
-          loss = fma(t, t, loss);
-        }
-      }
-    }
-  }
-  if (LOSS) {
-    double b = block_sum(loss, sh);
-    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = b;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Epilogue modes of the transposed product G = 2 F^T W + stencil.
 // ---------------------------------------------------------------------------
 enum TMode {
@@ -212,86 +116,164 @@ __device__ __forceinline__ double stencil_src(const double *__restrict__ E, cons
 }
 
 // ---------------------------------------------------------------------------
-// G[k,n] = 2 sum_d F[d,k] W[d,n] (+ 2 gamma second difference of E along k),
-// then in reduced space the fold G[k,:] -= G[k,maxr[k]] (pqdts.f90:334-340,
-// 1024-1030), then the active-set mask (pqdts.f90:2040-2052), plus a per-block
-// partial of sum dd*G (the CG's v2 = p.Hp, pqdts.f90:2058).
+// Band GEMM on FP64 MFMA: C[r, n] = sum_k A[r, k] B[k, n] for a banded A.
 //
-// Block = RK consecutive k x 256 consecutive n; the union d-range of the RK
-// columns of F is swept in chunks of KC staged in LDS as F^T.
+// Both products of the solver are this kernel:
+//   O = F V        A = F   (D x M), B = V (M x N)             (redO, pqdts.f90:724-736)
+//   G = 2 F^T W    A = F^T (M x D), B = W = O - P or O (D x N) (dL 998-1006, BF 280-288)
+// A is pre-packed per tile of TR rows as a dense, zero-filled, transposed band
+// [kchunks*KC][TR] covering the union of the tile's row spans, so staging a
+// chunk of A is one contiguous copy. B chunks (KC rows x TC columns, rows
+// contiguous along n) are staged through double-buffered LDS while the
+// previous chunk feeds v_mfma_f64_16x16x4f64. Four waves, each a 32 x 32
+// output tile (2 x 2 MFMA tiles, accumulators in registers).
 // ---------------------------------------------------------------------------
-template <int RK, int KC, int MODE, bool REDUCED, bool MASK, bool DOT>
-__global__ __launch_bounds__(BLOCK) void k_spmmT_cols(
-    const double *__restrict__ O, const double *__restrict__ P, int N, int M, int D,
-    const int *__restrict__ flo, const int *__restrict__ flen, const int64_t *__restrict__ foff,
-    const double *__restrict__ fval, const int *__restrict__ tlo, const int *__restrict__ thi, double gamma,
-    const double *__restrict__ E, const int *__restrict__ maxr, const double *__restrict__ ev,
-    const double *__restrict__ gm, const uint8_t *__restrict__ ik, const double *__restrict__ dd,
-    double *__restrict__ G, double *__restrict__ partial) {
-  __shared__ double Ft[RK][KC];
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+enum BMode { B_PLAIN = 0, B_EXPAND = 1, B_DIFF = 2 };
+enum EMode { E_STORE = 0, E_LOSS = 1, E_STORE_LOSS = 2, E_GRAD = 3, E_HVP = 4 };
+
+struct BandArgs {
+  const double *apack;    // packed A tiles
+  const int64_t *aoff;    // per tile: offset into apack
+  const int *klo;         // per tile: first k of the band
+  const int *kchunks;     // per tile: number of KC chunks
+  int nrows;              // rows of C
+  int nk;                 // rows of B
+  int N;                  // columns
+  const double *B;        // B operand (V or O)
+  const double *B2;       // P for B_DIFF
+  const int *maxr;        // implicit columns (B_EXPAND on rows of B; reduced epilogues on rows of C)
+  const double *ev;       // expansion values, indexed like maxr
+  double *C;              // output
+  const double *P;        // E_LOSS: data
+  const uint8_t *has;     // E_LOSS: rows of F that have entries
+  double gamma;           // E_GRAD / E_HVP: smoothing weight
+  const double *E;        // stencil source (X or the direction)
+  const double *gm;       // fold values G[k, maxr[k]]
+  const uint8_t *ik;      // active set
+  const double *dd;       // direction for the v2 partial
+  double *partial;        // per-block partials (loss or dot)
+};
+
+template <int TR, int TC, int KC, int BM, int EM, bool RED, bool MASK, bool DOT>
+__global__ __launch_bounds__(BLOCK) void k_band_gemm(BandArgs a) {
+  static_assert(TR == 64 && TC == 64, "wave layout assumes a 64 x 64 block tile");
+  constexpr int WC = 2;                  // waves along columns
+  constexpr int MR = 2, MC = 2;          // 16 x 16 MFMA tiles per wave along rows / columns
+  constexpr int AE = KC * TR / BLOCK;    // A elements staged per thread per chunk
+  constexpr int BE = KC * TC / BLOCK;    // B elements staged per thread per chunk
+  __shared__ double As[2][KC * TR];      // [kk][r]
+  __shared__ double Bs[2][KC * TC];      // [kk][c]
   __shared__ double sh[WAVES_PER_BLOCK];
   const int tile = blockIdx.y;
-  const int k0 = tile * RK;
-  const int n = blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = n < N;
-  double acc[RK];
+  const int r0 = tile * TR, n0 = blockIdx.x * TC;
+  const int tid = threadIdx.x, lane = lane_id(), w = wave_id();
+  const int wr0 = (w / WC) * 32, wc0 = (w % WC) * 32;
+  const double *ap = a.apack + a.aoff[tile];
+  const int k0 = a.klo[tile], nch = a.kchunks[tile];
+  const int N = a.N;
+
+  double ra[AE], rb[BE];
+  auto gload = [&](int c) {
+    const double *src = ap + (int64_t)c * (KC * TR);
 #pragma unroll
-  for (int r = 0; r < RK; r++) acc[r] = 0.0;
-  const int ulo = tlo[tile], uhi = thi[tile];
-  for (int c0 = ulo; c0 <= uhi; c0 += KC) {
-    const int kc = min(KC, uhi - c0 + 1);
-    for (int e = threadIdx.x; e < RK * KC; e += BLOCK) {
-      const int r = e / KC, dd_ = e - r * KC;
-      const int d = c0 + dd_;
+    for (int i = 0; i < AE; i++) ra[i] = src[tid + BLOCK * i];
+#pragma unroll
+    for (int i = 0; i < BE; i++) {
+      const int e = tid + BLOCK * i;
+      const int kk = e / TC, cc = e - kk * TC;
+      const int k = k0 + c * KC + kk, n = n0 + cc;
       double v = 0.0;
-      if (dd_ < kc) {
-        const int rel = k0 + r - flo[d];
-        if (rel >= 0 && rel < flen[d]) v = fval[foff[d] + rel];
+      if (k < a.nk && n < N) {
+        const int64_t idx = (int64_t)k * N + n;
+        v = a.B[idx];
+        if (BM == B_DIFF) v -= a.B2[idx];
+        if (BM == B_EXPAND && n == a.maxr[k]) v = a.ev[k];
       }
-      Ft[r][dd_] = v;
+      rb[i] = v;
     }
-    __syncthreads();
-    if (active) {
-      const double *op = O + (int64_t)c0 * N + n;
-      const double *pp = P + (int64_t)c0 * N + n;
-      for (int j = 0; j < kc; j++) {
-        double w = op[(int64_t)j * N];
-        if (MODE == T_GRAD) w -= pp[(int64_t)j * N];
+  };
+  auto sstore = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < RK; r++) acc[r] = fma(Ft[r][j], w, acc[r]);
-      }
+    for (int i = 0; i < AE; i++) As[buf][tid + BLOCK * i] = ra[i];
+#pragma unroll
+    for (int i = 0; i < BE; i++) Bs[buf][tid + BLOCK * i] = rb[i];
+  };
+
+  d4_t acc[MR][MC];
+#pragma unroll
+  for (int i = 0; i < MR; i++)
+#pragma unroll
+    for (int j = 0; j < MC; j++) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  if (nch > 0) {
+    gload(0);
+    sstore(0);
+  }
+  __syncthreads();
+  const int fk = lane >> 4, fi = lane & 15;  // fragment coordinates: k within the step, row/column
+  for (int c = 0; c < nch; c++) {
+    const int buf = c & 1;
+    if (c + 1 < nch) gload(c + 1);
+#pragma unroll
+    for (int ks = 0; ks < KC; ks += 4) {
+      double af[MR], bf[MC];
+#pragma unroll
+      for (int i = 0; i < MR; i++) af[i] = As[buf][(ks + fk) * TR + wr0 + i * 16 + fi];
+#pragma unroll
+      for (int j = 0; j < MC; j++) bf[j] = Bs[buf][(ks + fk) * TC + wc0 + j * 16 + fi];
+#pragma unroll
+      for (int i = 0; i < MR; i++)
+#pragma unroll
+        for (int j = 0; j < MC; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
+    if (c + 1 < nch) sstore(buf ^ 1);
     __syncthreads();
   }
-  double dot = 0.0;
-  if (active) {
-    const double g2 = 2.0 * gamma;
-    constexpr bool EXP = (MODE == T_HVP) && REDUCED;
+
+  // epilogue: register q of lane l holds row 4*q + l/16, column l%16 of each
+  // 16 x 16 tile (the FP64 MFMA output layout: four groups of one row)
+  double red = 0.0;
+  const double g2 = 2.0 * a.gamma;
+  constexpr bool EXP = (EM == E_HVP) && RED;
 #pragma unroll
-    for (int r = 0; r < RK; r++) {
-      const int k = k0 + r;
-      if (k >= M) break;
-      double g = 2.0 * acc[r];
-      if (gamma != 0.0) {
-        const double e0 = stencil_src<EXP>(E, maxr, ev, k, n, N);
-        if (k + 1 < M) g = g + g2 * (e0 - stencil_src<EXP>(E, maxr, ev, k + 1, n, N));
-        if (k > 0) g = g - g2 * (stencil_src<EXP>(E, maxr, ev, k - 1, n, N) - e0);
+  for (int i = 0; i < MR; i++)
+#pragma unroll
+    for (int j = 0; j < MC; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = r0 + wr0 + i * 16 + 4 * q + fk;
+        const int n = n0 + wc0 + j * 16 + fi;
+        if (row >= a.nrows || n >= N) continue;
+        const int64_t idx = (int64_t)row * N + n;
+        const double v = acc[i][j][q];
+        if (EM == E_STORE || EM == E_STORE_LOSS) a.C[idx] = v;
+        if ((EM == E_LOSS || EM == E_STORE_LOSS) && a.has[row]) {
+          const double t = v - a.P[idx];
+          red = fma(t, t, red);
+        }
+        if (EM == E_GRAD || EM == E_HVP) {
+          double g = 2.0 * v;
+          if (a.gamma != 0.0) {
+            const double e0 = stencil_src<EXP>(a.E, a.maxr, a.ev, row, n, N);
+            if (row + 1 < a.nrows) g = g + g2 * (e0 - stencil_src<EXP>(a.E, a.maxr, a.ev, row + 1, n, N));
+            if (row > 0) g = g - g2 * (stencil_src<EXP>(a.E, a.maxr, a.ev, row - 1, n, N) - e0);
+          }
+          if (RED) g = (n == a.maxr[row]) ? 0.0 : g - a.gm[row];
+          if (MASK && a.ik[idx]) g = 0.0;
+          a.C[idx] = g;
+          if (DOT) red = fma(a.dd[idx], g, red);
+        }
       }
-      const int64_t idx = (int64_t)k * N + n;
-      if (REDUCED) g = (n == maxr[k]) ? 0.0 : g - gm[k];
-      if (MASK && ik[idx]) g = 0.0;
-      G[idx] = g;
-      if (DOT) dot = fma(dd[idx], g, dot);
-    }
-  }
-  if (DOT) {
-    double b = block_sum(dot, sh);
-    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = b;
+  if (EM == E_LOSS || EM == E_STORE_LOSS || DOT) {
+    const double b = block_sum(red, sh);
+    if (threadIdx.x == 0) a.partial[blockIdx.y * gridDim.x + blockIdx.x] = b;
   }
 }
 
 // Pre-fold value G[k, maxr[k]] of the transposed product (one wave per k), used
-// by the fold of k_spmmT_cols. Same formula as k_spmmT_cols at n = maxr[k].
+// by the fold in the transposed band GEMM; same formula as its epilogue at n = maxr[k].
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_foldval(const double *__restrict__ O, const double *__restrict__ P,
                                                    int N, int M, const int *__restrict__ flo,

@@ -52,9 +52,8 @@ int guarded(F &&f) {
 }
 
 // tile heights of the two products
-constexpr int RD = 32;   // F rows per block of k_spmm_rows
-constexpr int RK = 32;   // G rows per block of k_spmmT_cols
-constexpr int KC = 64;   // staged chunk length
+constexpr int TILE = 64;  // rows and columns of a band-GEMM block tile
+constexpr int KC = 16;    // band chunk staged per LDS buffer (4 MFMA k-steps)
 
 // device scalar slots
 enum {
@@ -79,7 +78,7 @@ struct pqdts_ctx {
   int M = 0, N = 0, D = 0;
   int64_t MN = 0, DN = 0;
   int64_t nspan = 0;       // stored F elements: sum of row span lengths
-  int64_t tile_elems = 0;  // elements swept by the two products (union ranges x tile heights)
+  int64_t tile_elems = 0;  // packed elements of F's band tiles (the MFMA work of O = F V)
   double gamma = 0.0;
   int device = 0;
   hipStream_t s = nullptr;
@@ -88,8 +87,14 @@ struct pqdts_ctx {
   int64_t *foff = nullptr;
   double *fval = nullptr;
   uint8_t *fhas = nullptr;
+  // F (tiles of D rows) and F^T (tiles of M rows) packed as dense transposed bands
+  struct Band {
+    int ntiles = 0;
+    double *pack = nullptr;
+    int64_t *off = nullptr;
+    int *klo = nullptr, *nch = nullptr;
+  } bf, bt;
   int nrt = 0, nct = 0;
-  int *rt_lo = nullptr, *rt_hi = nullptr, *ct_lo = nullptr, *ct_hi = nullptr;
   int *kd_lo = nullptr, *kd_hi = nullptr;
   double *P = nullptr, *O = nullptr;
   double *slot[6] = {};
@@ -237,29 +242,45 @@ void op_reconstruct(pqdts_ctx *c, double *X) {
   k_reconstruct<<<c->row_blocks(), BLOCK, 0, c->s>>>(X, c->M, c->N, c->maxr);
 }
 
-// O = F V; returns the number of LOSS partials written to region R_A
+BandArgs band_args(const pqdts_ctx::Band &b) {
+  BandArgs a{};
+  a.apack = b.pack;
+  a.aoff = b.off;
+  a.klo = b.klo;
+  a.kchunks = b.nch;
+  return a;
+}
+
+// O = F V (redO, pqdts.f90:724-736); returns the number of LOSS partials in R_A
 int op_spmm(pqdts_ctx *c, const double *V, bool expand, bool loss, bool store) {
-  dim3 grid(cdiv(c->N, BLOCK), c->nrt);
-  double *part = c->reg(R_A);
+  BandArgs a = band_args(c->bf);
+  a.nrows = c->D;
+  a.nk = c->M;
+  a.N = c->N;
+  a.B = V;
+  a.maxr = c->maxr;
+  a.ev = c->ev;
+  a.C = c->O;
+  a.P = c->P;
+  a.has = c->fhas;
+  a.partial = c->reg(R_A);
+  dim3 grid(cdiv(c->N, TILE), c->bf.ntiles);
   hipEvent_t ev0 = c->kbegin();
-#define SPMM_ARGS                                                                                      \
-  V, c->N, c->D, c->flo, c->flen, c->foff, c->fval, c->fhas, c->rt_lo, c->rt_hi, c->maxr, c->ev, c->P, \
-      c->O, part
   if (expand)
-    k_spmm_rows<RD, KC, true, false, true><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
+    k_band_gemm<TILE, TILE, KC, B_EXPAND, E_STORE, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
   else if (loss && store)
-    k_spmm_rows<RD, KC, false, true, true><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
+    k_band_gemm<TILE, TILE, KC, B_PLAIN, E_STORE_LOSS, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
   else if (loss)
-    k_spmm_rows<RD, KC, false, true, false><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
+    k_band_gemm<TILE, TILE, KC, B_PLAIN, E_LOSS, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
   else
-    k_spmm_rows<RD, KC, false, false, true><<<grid, BLOCK, 0, c->s>>>(SPMM_ARGS);
-#undef SPMM_ARGS
+    k_band_gemm<TILE, TILE, KC, B_PLAIN, E_STORE, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
   HIPC(hipGetLastError());
   c->kend(ev0, "spmm_F");
   return grid.x * grid.y;
 }
 
-// G = 2 F^T W + stencil (+ fold / mask / dot); returns #DOT partials in R_B
+// G = 2 F^T W + stencil (+ fold / mask / dot) (dL 998-1030, BF 276-340);
+// returns the number of DOT partials in R_B
 int op_spmmT(pqdts_ctx *c, int mode, const double *E, const double *dd, double *G, bool mask, bool dot) {
   const bool red = !c->full;
   if (red) need_maxr(c);
@@ -273,31 +294,43 @@ int op_spmmT(pqdts_ctx *c, int mode, const double *E, const double *dd, double *
                                                            c->fval, c->kd_lo, c->kd_hi, c->gamma, E, c->maxr,
                                                            c->ev, c->gm);
   }
-  dim3 grid(cdiv(c->N, BLOCK), c->nct);
-  double *part = c->reg(R_B);
+  BandArgs a = band_args(c->bt);
+  a.nrows = c->M;
+  a.nk = c->D;
+  a.N = c->N;
+  a.B = c->O;
+  a.B2 = c->P;
+  a.maxr = c->maxr;
+  a.ev = c->ev;
+  a.C = G;
+  a.gamma = c->gamma;
+  a.E = E;
+  a.gm = c->gm;
+  a.ik = c->ik;
+  a.dd = dd;
+  a.partial = c->reg(R_B);
+  dim3 grid(cdiv(c->N, TILE), c->bt.ntiles);
   hipEvent_t ev0 = c->kbegin();
-#define SPMMT_ARGS                                                                                        \
-  c->O, c->P, c->N, c->M, c->D, c->flo, c->flen, c->foff, c->fval, c->ct_lo, c->ct_hi, c->gamma, E, c->maxr, \
-      c->ev, c->gm, c->ik, dd, G, part
+#define BG(BM, EM, R, MK, DT) k_band_gemm<TILE, TILE, KC, BM, EM, R, MK, DT><<<grid, BLOCK, 0, c->s>>>(a)
   if (mode == T_GRAD) {
     if (red)
-      k_spmmT_cols<RK, KC, T_GRAD, true, false, false><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+      BG(B_DIFF, E_GRAD, true, false, false);
     else
-      k_spmmT_cols<RK, KC, T_GRAD, false, false, false><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+      BG(B_DIFF, E_GRAD, false, false, false);
   } else {
     if (red) {
       if (mask && dot)
-        k_spmmT_cols<RK, KC, T_HVP, true, true, true><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+        BG(B_PLAIN, E_HVP, true, true, true);
       else
-        k_spmmT_cols<RK, KC, T_HVP, true, false, false><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+        BG(B_PLAIN, E_HVP, true, false, false);
     } else {
       if (mask && dot)
-        k_spmmT_cols<RK, KC, T_HVP, false, true, true><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+        BG(B_PLAIN, E_HVP, false, true, true);
       else
-        k_spmmT_cols<RK, KC, T_HVP, false, false, false><<<grid, BLOCK, 0, c->s>>>(SPMMT_ARGS);
+        BG(B_PLAIN, E_HVP, false, false, false);
     }
   }
-#undef SPMMT_ARGS
+#undef BG
   HIPC(hipGetLastError());
   c->kend(ev0, "spmmT_F");
   return grid.x * grid.y;
@@ -571,9 +604,10 @@ void destroy_ctx(pqdts_ctx *c) {
     (void)hipEventDestroy(p.b);
   }
   for (hipEvent_t e : c->kpool) (void)hipEventDestroy(e);
-  void *ptrs[] = {c->flo,  c->flen, c->foff, c->fval,      c->fhas,     c->rt_lo, c->rt_hi, c->ct_lo,
-                  c->ct_hi, c->kd_lo, c->kd_hi, c->P,        c->O,        c->diag_full, c->diag_red, c->maxr,
-                  c->ik,   c->ev,   c->gm,   c->lam,       c->lmin,     c->part,  c->scal};
+  void *ptrs[] = {c->flo,      c->flen,      c->foff,   c->fval,   c->fhas,   c->bf.pack, c->bf.off,
+                  c->bf.klo,   c->bf.nch,    c->bt.pack, c->bt.off, c->bt.klo, c->bt.nch,  c->kd_lo,
+                  c->kd_hi,    c->P,         c->O,      c->diag_full, c->diag_red, c->maxr, c->ik,
+                  c->ev,       c->gm,        c->lam,    c->lmin,   c->part,   c->scal};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (double *p : c->slot)
@@ -581,6 +615,62 @@ void destroy_ctx(pqdts_ctx *c) {
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
+}
+
+// Packs the rows of a banded matrix into per-tile dense transposed bands for
+// k_band_gemm: tile t covers rows [t*TILE, (t+1)*TILE); its band starts at
+// column klo[t] and holds nch[t]*KC zero-filled columns; element (row r, col k)
+// of the tile sits at off[t] + (k - klo[t])*TILE + (r - t*TILE).
+struct HostBand {
+  std::vector<double> pack;
+  std::vector<int64_t> off;
+  std::vector<int> klo, nch;
+};
+
+template <class Span, class Val>
+HostBand pack_band(int R, Span span, Val val) {
+  const int nt = cdiv(R, TILE);
+  HostBand b;
+  b.off.assign(nt, 0);
+  b.klo.assign(nt, 0);
+  b.nch.assign(nt, 0);
+  int64_t total = 0;
+  for (int t = 0; t < nt; t++) {
+    int lo = INT32_MAX, hi = -1;
+    for (int r = t * TILE; r < std::min(R, (t + 1) * TILE); r++) {
+      const auto sp = span(r);
+      if (sp.second > 0) {
+        lo = std::min(lo, sp.first);
+        hi = std::max(hi, sp.first + sp.second - 1);
+      }
+    }
+    b.off[t] = total;
+    if (hi >= 0) {
+      b.klo[t] = lo;
+      b.nch[t] = cdiv(hi - lo + 1, KC);
+      total += (int64_t)b.nch[t] * KC * TILE;
+    }
+  }
+  b.pack.assign((size_t)std::max<int64_t>(total, 1), 0.0);
+  for (int t = 0; t < nt; t++)
+    for (int r = t * TILE; r < std::min(R, (t + 1) * TILE); r++) {
+      const auto sp = span(r);
+      for (int k = sp.first; k < sp.first + sp.second; k++)
+        b.pack[b.off[t] + (int64_t)(k - b.klo[t]) * TILE + (r - t * TILE)] = val(r, k);
+    }
+  return b;
+}
+
+void upload_band(pqdts_ctx *c, pqdts_ctx::Band &dst, const HostBand &h) {
+  dst.ntiles = (int)h.off.size();
+  dst.pack = dalloc<double>((int64_t)h.pack.size());
+  dst.off = dalloc<int64_t>(dst.ntiles);
+  dst.klo = dalloc<int>(dst.ntiles);
+  dst.nch = dalloc<int>(dst.ntiles);
+  HIPC(hipMemcpyAsync(dst.pack, h.pack.data(), sizeof(double) * h.pack.size(), hipMemcpyHostToDevice, c->s));
+  HIPC(hipMemcpyAsync(dst.off, h.off.data(), sizeof(int64_t) * dst.ntiles, hipMemcpyHostToDevice, c->s));
+  HIPC(hipMemcpyAsync(dst.klo, h.klo.data(), sizeof(int) * dst.ntiles, hipMemcpyHostToDevice, c->s));
+  HIPC(hipMemcpyAsync(dst.nch, h.nch.data(), sizeof(int) * dst.ntiles, hipMemcpyHostToDevice, c->s));
 }
 
 pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *Fr, const int32_t *Fc,
@@ -613,22 +703,7 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
   }
   std::vector<double> val(off[D], 0.0);
   for (int64_t e = 0; e < nF; e++) val[off[Fr[e]] + (Fc[e] - lo[Fr[e]])] = Fv[e];
-  // tiles of k_spmm_rows: union k-range of RD rows
-  const int nrt = cdiv(D, RD);
-  std::vector<int> rtlo(nrt, 0), rthi(nrt, -1);
-  for (int t = 0; t < nrt; t++) {
-    int a = INT32_MAX, b = -1;
-    for (int d = t * RD; d < std::min<int64_t>(D, (int64_t)(t + 1) * RD); d++)
-      if (has[d]) {
-        a = std::min(a, lo[d]);
-        b = std::max(b, lo[d] + len[d] - 1);
-      }
-    if (b >= 0) {
-      rtlo[t] = a;
-      rthi[t] = b;
-    }
-  }
-  // per-k d-range and tiles of k_spmmT_cols
+  // per-k d-range (column spans of F)
   std::vector<int> kdlo(M, INT32_MAX), kdhi(M, -1);
   for (int d = 0; d < D; d++)
     for (int k = lo[d]; k < lo[d] + len[d]; k++) {
@@ -637,20 +712,16 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     }
   for (int k = 0; k < M; k++)
     if (kdhi[k] < 0) kdlo[k] = 0;
-  const int nct = cdiv(M, RK);
-  std::vector<int> ctlo(nct, 0), cthi(nct, -1);
-  for (int t = 0; t < nct; t++) {
-    int a = INT32_MAX, b = -1;
-    for (int k = t * RK; k < std::min<int64_t>(M, (int64_t)(t + 1) * RK); k++)
-      if (kdhi[k] >= 0) {
-        a = std::min(a, kdlo[k]);
-        b = std::max(b, kdhi[k]);
-      }
-    if (b >= 0) {
-      ctlo[t] = a;
-      cthi[t] = b;
-    }
-  }
+  // F and F^T packed for the band GEMM
+  HostBand hbf = pack_band(
+      (int)D, [&](int d) { return std::make_pair(lo[d], len[d]); },
+      [&](int d, int k) { return val[off[d] + (k - lo[d])]; });
+  HostBand hbt = pack_band(
+      (int)M, [&](int k) { return std::make_pair(kdlo[k], kdhi[k] >= 0 ? kdhi[k] - kdlo[k] + 1 : 0); },
+      [&](int k, int d) {
+        const int rel = k - lo[d];
+        return (rel >= 0 && rel < len[d]) ? val[off[d] + rel] : 0.0;
+      });
   // P densify (pqdts.f90:1527-1535), last duplicate wins
   std::vector<double> Ph((size_t)(D * N), 0.0);
   for (int64_t e = 0; e < nP; e++) {
@@ -669,19 +740,15 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     c->sync_timers = getenv("PQDTS_SYNC_TIMERS") && atoi(getenv("PQDTS_SYNC_TIMERS")) != 0;
     HIPC(hipGetDevice(&c->device));
     HIPC(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-    c->nrt = nrt;
     c->nspan = off[D];
-    for (int t = 0; t < nrt; t++) c->tile_elems += (int64_t)RD * std::max(0, rthi[t] - rtlo[t] + 1);
-    c->nct = nct;
+    c->tile_elems = (int64_t)hbf.pack.size();
+    c->nrt = (int)hbf.off.size();
+    c->nct = (int)hbt.off.size();
     c->flo = dalloc<int>(D);
     c->flen = dalloc<int>(D);
     c->foff = dalloc<int64_t>(D);
     c->fval = dalloc<double>(off[D]);
     c->fhas = dalloc<uint8_t>(D);
-    c->rt_lo = dalloc<int>(nrt);
-    c->rt_hi = dalloc<int>(nrt);
-    c->ct_lo = dalloc<int>(nct);
-    c->ct_hi = dalloc<int>(nct);
     c->kd_lo = dalloc<int>(M);
     c->kd_hi = dalloc<int>(M);
     c->P = dalloc<double>(c->DN);
@@ -695,8 +762,8 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     c->gm = dalloc<double>(M);
     c->lam = dalloc<double>(M);
     c->lmin = dalloc<double>(M);
-    const int gx = cdiv(N, BLOCK);
-    c->pcap = std::max({gx * nrt, gx * nct, cdiv(M, WAVES_PER_BLOCK), 4096});
+    const int gx = cdiv(N, TILE);
+    c->pcap = std::max({gx * c->nrt, gx * c->nct, cdiv(M, WAVES_PER_BLOCK), 4096});
     c->part = dalloc<double>((int64_t)c->pcap * R_NREG);
     c->scal = dalloc<double>(S_NSCAL);
     HIPC(hipHostMalloc((void **)&c->hscal, sizeof(double) * S_NSCAL, hipHostMallocDefault));
@@ -708,10 +775,8 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     up(c->foff, off.data(), sizeof(int64_t) * D);
     up(c->fval, val.data(), sizeof(double) * off[D]);
     up(c->fhas, has.data(), D);
-    up(c->rt_lo, rtlo.data(), sizeof(int) * nrt);
-    up(c->rt_hi, rthi.data(), sizeof(int) * nrt);
-    up(c->ct_lo, ctlo.data(), sizeof(int) * nct);
-    up(c->ct_hi, cthi.data(), sizeof(int) * nct);
+    upload_band(c, c->bf, hbf);
+    upload_band(c, c->bt, hbt);
     up(c->kd_lo, kdlo.data(), sizeof(int) * M);
     up(c->kd_hi, kdhi.data(), sizeof(int) * M);
     up(c->P, Ph.data(), sizeof(double) * c->DN);
