@@ -171,7 +171,10 @@ def main():
     dom = max(kstats, key=lambda k: kstats[k][1])
     launches, ms = kstats[dom]
     avg_s = ms / 1e3 / launches
-    flops = 2.0 * info["nspan"] * prob.N
+    # algorithmic FP64 work per launch: stored elements of the banded operand x N columns x 2
+    work = {"spmm_F": 2.0 * info["nspan"] * prob.N, "spmmT_F": 2.0 * info["nspan"] * prob.N,
+            "hvp_G": 2.0 * info["gram_nnz"] * prob.N}
+    flops = work[dom]
     achieved = flops / avg_s / 1e12
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom,
@@ -206,7 +209,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cfg2 plain FP64 solve, one problem at a time", "M": prob.M, "N": prob.N,
                    "D": prob.D, "nnzF": prob.nnz, "gamma": prob.gamma, "eps": prob.eps, "mode": 2,
-                   "eta": 0.75, "shots": 1e7, "npix": 1000},
+                   "eta": 0.75, "shots": 1e7, "npix": 1000,
+                   "hvp": "gram" if info["gram"] else "F products"},
         "latency_s": {"p50": float(np.percentile(lat_sorted, 50)), "p99": float(np.percentile(lat_sorted, 99)),
                       "max": float(lat_sorted[-1])},
         "solve": {"newton_iterations": len(its), "pcg_iterations": npcg, "line_search_trials": nls,

@@ -150,10 +150,18 @@ int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double
 int pqdts_kernel_timing(pqdts_ctx *ctx, int on);
 int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms);
 
-/* Sizes of a context: info[0..7] = M, N, D, stored F elements (sum of row span
- * lengths, the reference's Frows storage), elements swept by the F*V tiles,
- * #row tiles, #column tiles, 0. */
+/* Sizes of a context: info[0..11] = M, N, D, stored F elements (sum of row span
+ * lengths, the reference's Frows storage), packed elements of F's band tiles,
+ * #F tiles, #F^T tiles, Gram path active (0/1), packed elements of G's band
+ * tiles, stored elements of G's row bands, 0, 0. */
 int pqdts_info(pqdts_ctx *ctx, int64_t *info);
+
+/* How BF (and every CG step) applies the Hessian: 1 = the reference's two
+ * products 2 F^T (F p) (pqdts.f90:272-288); 2 = one product with the banded Gram
+ * matrix G = F^T F, formed once on the GPU; 0 = automatic (G when its band tiles
+ * cost less than 0.8 of the two products). Same operator, different rounding.
+ * No counterpart in the reference. */
+int pqdts_set_hvp_path(pqdts_ctx *ctx, int path);
 
 /* simplexproj_condat_ (condat_simplexproj.c:60) over many rows at once: rows of
  * length n from y (host, row-major) projected onto the simplex into x (host).

@@ -26,7 +26,7 @@ EXPORTS = [
     "pqdts_conv_test", "pqdts_pdot", "pqdts_paxpy", "pqdts_pgaxpy", "pqdts_pgaxpy2", "pqdts_pset",
     "pqdts_pcopy", "pqdts_pscale", "pqdts_pcopy_scale", "pqdts_transform", "pqdts_lalpha_min2",
     "pqdts_lalpha_min_auglag", "pqdts_solve", "pqdts_timers", "pqdts_simplexproj_rows",
-    "pqdts_kernel_timing", "pqdts_kernel_stats", "pqdts_info",
+    "pqdts_kernel_timing", "pqdts_kernel_stats", "pqdts_info", "pqdts_set_hvp_path",
 ]
 
 _lib = None
@@ -81,6 +81,7 @@ def load():
     L.pqdts_kernel_timing.argtypes = [vp, i32]
     L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
+    L.pqdts_set_hvp_path.argtypes = [vp, i32]
     _lib = L
     return L
 
@@ -270,10 +271,14 @@ class Context:
         return conv.value, events
 
     def info(self) -> dict:
-        a = (C.c_int64 * 8)()
+        a = (C.c_int64 * 12)()
         _check(load().pqdts_info(self.h, a))
         return {"M": a[0], "N": a[1], "D": a[2], "nspan": a[3], "tile_elems": a[4], "row_tiles": a[5],
-                "col_tiles": a[6]}
+                "col_tiles": a[6], "gram": bool(a[7]), "gram_tile_elems": a[8], "gram_nnz": a[9]}
+
+    def set_hvp_path(self, path):
+        """0 automatic, 1 the two F products, 2 the Gram matrix G = F^T F."""
+        _check(load().pqdts_set_hvp_path(self.h, {"auto": 0, "F": 1, "G": 2}.get(path, path)))
 
     def kernel_timing(self, on: bool):
         _check(load().pqdts_kernel_timing(self.h, int(bool(on))))

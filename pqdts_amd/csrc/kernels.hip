@@ -308,6 +308,56 @@ __global__ __launch_bounds__(BLOCK) void k_foldval(const double *__restrict__ O,
   }
 }
 
+// Gram path of BF: pre-fold value (2 G p~ + stencil)[k, maxr[k]] from the packed
+// band of G = F^T F (layout of k_band_gemm's A operand), one wave per k. p~ is
+// the reduced-space expansion of dd (implicit column replaced by ev).
+__global__ __launch_bounds__(BLOCK) void k_foldval_gram(const double *__restrict__ gpack,
+                                                        const int64_t *__restrict__ goff,
+                                                        const int *__restrict__ glo, const int *__restrict__ gnch,
+                                                        int kc, int tile, int N, int M, double gamma,
+                                                        const double *__restrict__ dd, const int *__restrict__ maxr,
+                                                        const double *__restrict__ ev, double *__restrict__ gm) {
+  const int k = blockIdx.x * WAVES_PER_BLOCK + wave_id();
+  if (k >= M) return;
+  const int c = maxr[k];
+  const int t = k / tile, r = k - t * tile;
+  const int l0 = glo[t], nl = gnch[t] * kc;
+  const double *gp = gpack + goff[t] + r;
+  double acc = 0.0;
+  for (int j = lane_id(); j < nl; j += WAVE) {
+    const int l = l0 + j;
+    if (l >= M) break;
+    const double p = (c == maxr[l]) ? ev[l] : dd[(int64_t)l * N + c];
+    acc = fma(gp[(int64_t)j * tile], p, acc);
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0) {
+    double g = 2.0 * acc;
+    if (gamma != 0.0) {
+      const double g2 = 2.0 * gamma;
+      const double e0 = stencil_src<true>(dd, maxr, ev, k, c, N);
+      if (k + 1 < M) g = g + g2 * (e0 - stencil_src<true>(dd, maxr, ev, k + 1, c, N));
+      if (k > 0) g = g - g2 * (stencil_src<true>(dd, maxr, ev, k - 1, c, N) - e0);
+    }
+    gm[k] = g;
+  }
+}
+
+// Dense copy of F (D x M, row-major) from its row spans, one wave per row; used
+// once to form the Gram matrix G = F^T F with the band GEMM.
+__global__ __launch_bounds__(BLOCK) void k_dense_rows(const int *__restrict__ flo, const int *__restrict__ flen,
+                                                      const int64_t *__restrict__ foff,
+                                                      const double *__restrict__ fval, int D, int M,
+                                                      double *__restrict__ Fd) {
+  const int d = blockIdx.x * WAVES_PER_BLOCK + wave_id();
+  if (d >= D) return;
+  double *row = Fd + (int64_t)d * M;
+  for (int k = lane_id(); k < M; k += WAVE) {
+    const int rel = k - flo[d];
+    row[k] = (rel >= 0 && rel < flen[d]) ? fval[foff[d] + rel] : 0.0;
+  }
+}
+
 // Bdiag (pqdts.f90:208-244), one wave per k: 2 sum_d F[d,k]^2 + 2 gamma [k<M-1]
 // + 2 gamma [k>0]; the reduced-space copy is twice that (pqdts.f90:235-236).
 __global__ __launch_bounds__(BLOCK) void k_bdiag(int M, const int *__restrict__ flo, const int *__restrict__ flen,

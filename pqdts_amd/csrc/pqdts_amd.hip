@@ -93,8 +93,14 @@ struct pqdts_ctx {
     double *pack = nullptr;
     int64_t *off = nullptr;
     int *klo = nullptr, *nch = nullptr;
-  } bf, bt;
+  } bf, bt, bg;  // bg: the Gram matrix G = F^T F (M x M), built on demand
   int nrt = 0, nct = 0;
+  // HVP path: the two products through O = F p (F path) or one product with G
+  int hvp_pref = 0;           // 0 auto (cost model), 1 F path, 2 Gram path
+  bool gram_auto = false;     // what the cost model chose at create
+  int64_t gram_elems = 0;     // packed elements of G's band tiles (0 until known)
+  std::vector<int> glo, gw;   // G's row bands on the host (start, width)
+  bool use_gram() const { return hvp_pref == 2 || (hvp_pref == 0 && gram_auto); }
   int *kd_lo = nullptr, *kd_hi = nullptr;
   double *P = nullptr, *O = nullptr;
   double *slot[6] = {};
@@ -368,11 +374,58 @@ void op_dL(pqdts_ctx *c, double *X, double *G) {
 
 // Hessian times dd (pqdts.f90:246-342). `have_ev`: ev already holds the
 // expansion of dd (computed by the CG kernels).
+void ensure_gram(pqdts_ctx *c);
+
+// Gram path of BF: BX = 2 G p~ + stencil (+ fold / mask / dot) with G = F^T F,
+// mathematically the same operator as 2 F^T (F p~) + stencil; no O is formed.
+int op_BF_gram(pqdts_ctx *c, const double *dd, double *BX, bool mask, bool dot) {
+  const bool red = !c->full;
+  if (red)
+    k_foldval_gram<<<c->row_blocks(), BLOCK, 0, c->s>>>(c->bg.pack, c->bg.off, c->bg.klo, c->bg.nch, KC, TILE,
+                                                         c->N, c->M, c->gamma, dd, c->maxr, c->ev, c->gm);
+  BandArgs a = band_args(c->bg);
+  a.nrows = c->M;
+  a.nk = c->M;
+  a.N = c->N;
+  a.B = dd;
+  a.maxr = c->maxr;
+  a.ev = c->ev;
+  a.C = BX;
+  a.gamma = c->gamma;
+  a.E = dd;
+  a.gm = c->gm;
+  a.ik = c->ik;
+  a.dd = dd;
+  a.partial = c->reg(R_B);
+  dim3 grid(cdiv(c->N, TILE), c->bg.ntiles);
+  hipEvent_t ev0 = c->kbegin();
+#define BG(BM, R, MK, DT) k_band_gemm<TILE, TILE, KC, BM, E_HVP, R, MK, DT><<<grid, BLOCK, 0, c->s>>>(a)
+  if (red) {
+    if (mask && dot)
+      BG(B_EXPAND, true, true, true);
+    else
+      BG(B_EXPAND, true, false, false);
+  } else {
+    if (mask && dot)
+      BG(B_PLAIN, false, true, true);
+    else
+      BG(B_PLAIN, false, false, false);
+  }
+#undef BG
+  HIPC(hipGetLastError());
+  c->kend(ev0, "hvp_G");
+  return grid.x * grid.y;
+}
+
 int op_BF(pqdts_ctx *c, const double *dd, double *BX, bool have_ev, bool mask, bool dot) {
   Scope t(c, "BF");
   if (!c->full) need_maxr(c);
   if (!c->full && !have_ev)
     k_others<<<c->row_blocks(), BLOCK, 0, c->s>>>(dd, c->M, c->N, c->maxr, c->ev);
+  if (c->use_gram()) {
+    ensure_gram(c);
+    return op_BF_gram(c, dd, BX, mask, dot);
+  }
   op_spmm(c, dd, !c->full, false, true);
   return op_spmmT(c, T_HVP, dd, dd, BX, mask, dot);
 }
@@ -605,7 +658,8 @@ void destroy_ctx(pqdts_ctx *c) {
   }
   for (hipEvent_t e : c->kpool) (void)hipEventDestroy(e);
   void *ptrs[] = {c->flo,      c->flen,      c->foff,   c->fval,   c->fhas,   c->bf.pack, c->bf.off,
-                  c->bf.klo,   c->bf.nch,    c->bt.pack, c->bt.off, c->bt.klo, c->bt.nch,  c->kd_lo,
+                  c->bf.klo,   c->bf.nch,    c->bt.pack, c->bt.off, c->bt.klo, c->bt.nch,  c->bg.pack,
+                  c->bg.off,   c->bg.klo,    c->bg.nch,  c->kd_lo,
                   c->kd_hi,    c->P,         c->O,      c->diag_full, c->diag_red, c->maxr, c->ik,
                   c->ev,       c->gm,        c->lam,    c->lmin,   c->part,   c->scal};
   for (void *p : ptrs)
@@ -671,6 +725,61 @@ void upload_band(pqdts_ctx *c, pqdts_ctx::Band &dst, const HostBand &h) {
   HIPC(hipMemcpyAsync(dst.off, h.off.data(), sizeof(int64_t) * dst.ntiles, hipMemcpyHostToDevice, c->s));
   HIPC(hipMemcpyAsync(dst.klo, h.klo.data(), sizeof(int) * dst.ntiles, hipMemcpyHostToDevice, c->s));
   HIPC(hipMemcpyAsync(dst.nch, h.nch.data(), sizeof(int) * dst.ntiles, hipMemcpyHostToDevice, c->s));
+}
+
+// Packed elements of the band tiles of a banded matrix (the MFMA work per output column).
+template <class Span>
+int64_t band_elems(int R, Span span) {
+  int64_t total = 0;
+  for (int t = 0; t < cdiv(R, TILE); t++) {
+    int lo = INT32_MAX, hi = -1;
+    for (int r = t * TILE; r < std::min(R, (t + 1) * TILE); r++) {
+      const auto sp = span(r);
+      if (sp.second > 0) {
+        lo = std::min(lo, sp.first);
+        hi = std::max(hi, sp.first + sp.second - 1);
+      }
+    }
+    if (hi >= 0) total += (int64_t)cdiv(hi - lo + 1, KC) * KC * TILE;
+  }
+  return total;
+}
+
+// Forms G = F^T F once, on the device: F is expanded to a dense D x M matrix and
+// multiplied by the packed F^T band with the band GEMM itself; the rows of G are
+// then packed into band tiles like F.
+void ensure_gram(pqdts_ctx *c) {
+  if (c->bg.pack) return;
+  const int M = c->M, D = c->D;
+  double *Fd = nullptr, *Gd = nullptr;
+  try {
+    Fd = dalloc<double>((int64_t)D * M);
+    Gd = dalloc<double>((int64_t)M * M);
+    k_dense_rows<<<cdiv(D, WAVES_PER_BLOCK), BLOCK, 0, c->s>>>(c->flo, c->flen, c->foff, c->fval, D, M, Fd);
+    BandArgs a = band_args(c->bt);
+    a.nrows = M;
+    a.nk = D;
+    a.N = M;
+    a.B = Fd;
+    a.C = Gd;
+    dim3 grid(cdiv(M, TILE), c->bt.ntiles);
+    k_band_gemm<TILE, TILE, KC, B_PLAIN, E_STORE, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
+    HIPC(hipGetLastError());
+    std::vector<double> G((size_t)M * M);
+    HIPC(hipMemcpyAsync(G.data(), Gd, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->s));
+    HIPC(hipStreamSynchronize(c->s));
+    HostBand h = pack_band(
+        M, [&](int k) { return std::make_pair(c->glo[k], c->gw[k]); },
+        [&](int k, int l) { return G[(size_t)k * M + l]; });
+    upload_band(c, c->bg, h);
+    HIPC(hipStreamSynchronize(c->s));
+  } catch (...) {
+    if (Fd) (void)hipFree(Fd);
+    if (Gd) (void)hipFree(Gd);
+    throw;
+  }
+  (void)hipFree(Fd);
+  (void)hipFree(Gd);
 }
 
 pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *Fr, const int32_t *Fc,
@@ -744,6 +853,27 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     c->tile_elems = (int64_t)hbf.pack.size();
     c->nrt = (int)hbf.off.size();
     c->nct = (int)hbt.off.size();
+    // G = F^T F is banded: row k reaches from the first to the last column of
+    // every F row that covers k. The Gram HVP is chosen when its band tiles are
+    // clearly cheaper than the two F products (and G fits in HBM many times over).
+    c->glo.assign(M, INT32_MAX);
+    c->gw.assign(M, 0);
+    {
+      std::vector<int> ghi(M, -1);
+      for (int d = 0; d < D; d++)
+        for (int k = lo[d]; k < lo[d] + len[d]; k++) {
+          c->glo[k] = std::min(c->glo[k], lo[d]);
+          ghi[k] = std::max(ghi[k], lo[d] + len[d] - 1);
+        }
+      for (int k = 0; k < M; k++) {
+        if (ghi[k] < 0) c->glo[k] = 0;
+        c->gw[k] = ghi[k] >= 0 ? ghi[k] - c->glo[k] + 1 : 0;
+      }
+    }
+    c->gram_elems = band_elems((int)M, [&](int k) { return std::make_pair(c->glo[k], c->gw[k]); });
+    const int64_t fpath = (int64_t)hbf.pack.size() + (int64_t)hbt.pack.size();
+    c->gram_auto = (double)c->gram_elems < 0.8 * (double)fpath && M * M * 8 < ((int64_t)64 << 30);
+    if (const char *e = getenv("PQDTS_HVP")) c->hvp_pref = (e[0] == 'G') ? 2 : (e[0] == 'F') ? 1 : 0;
     c->flo = dalloc<int>(D);
     c->flen = dalloc<int>(D);
     c->foff = dalloc<int64_t>(D);
@@ -1091,7 +1221,22 @@ int pqdts_info(pqdts_ctx *ctx, int64_t *info) {
     info[4] = ctx->tile_elems;
     info[5] = ctx->nrt;
     info[6] = ctx->nct;
-    info[7] = 0;
+    info[7] = ctx->use_gram() ? 1 : 0;
+    info[8] = ctx->gram_elems;
+    int64_t gnnz = 0;
+    for (int w : ctx->gw) gnnz += w;
+    info[9] = gnnz;
+    info[10] = 0;
+    info[11] = 0;
+  });
+}
+
+int pqdts_set_hvp_path(pqdts_ctx *ctx, int path) {
+  return guarded([&] {
+    checked(ctx);
+    require(path >= 0 && path <= 2, "hvp path must be 0 (auto), 1 (F products) or 2 (Gram)");
+    ctx->hvp_pref = path;
+    if (ctx->use_gram()) ensure_gram(ctx);
   });
 }
 

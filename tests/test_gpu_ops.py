@@ -67,9 +67,12 @@ def test_gpu_visible():
     assert pqdts_amd.device_count() >= 1
 
 
+@pytest.mark.parametrize("path", ["F", "G"])
 @pytest.mark.parametrize("full", [True, False])
-def test_L_dL_BF(pair, full):
+def test_L_dL_BF(pair, full, path):
+    """path F: BF as 2 F^T (F p); path G: BF with the banded Gram matrix F^T F."""
     ctx, orc, M, N = pair
+    ctx.set_hvp_path(path)
     rng = np.random.default_rng(1)
     set_both(ctx, orc, "X", simplex_rows(rng, M, N))
     if not full:
@@ -85,13 +88,14 @@ def test_L_dL_BF(pair, full):
     ctx.dL("X", "G")
     orc.dL("X", "G")
     close_arr(ctx.get("G"), orc.buf("G"))
+    # O = F X as left by the gradient
+    close_arr(ctx.get_O(), orc.O)
     V = rng.normal(size=(M, N))
     set_both(ctx, orc, "XTMP", V)
     ctx.BF("XTMP", "HP")
     orc.BF("XTMP", "HP")
     close_arr(ctx.get("HP"), orc.buf("HP"))
-    # O = F X as left by the last product
-    close_arr(ctx.get_O(), orc.O)
+    assert ctx.info()["gram"] == (path == "G")
     orc.Bdiag()
     close_arr(ctx.Bdiag(), orc.diag)
 
