@@ -153,7 +153,8 @@ int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, 
 /* Sizes of a context: info[0..11] = M, N, D, stored F elements (sum of row span
  * lengths, the reference's Frows storage), packed elements of F's band tiles,
  * #F tiles, #F^T tiles, Gram path active (0/1), packed elements of G's band
- * tiles, stored elements of G's row bands, 0, 0. */
+ * tiles, stored elements of G's row bands, packed elements of F^T's band
+ * tiles, device bytes held by the problem. */
 int pqdts_info(pqdts_ctx *ctx, int64_t *info);
 
 /* How BF (and every CG step) applies the Hessian: 1 = the reference's two
@@ -162,6 +163,35 @@ int pqdts_info(pqdts_ctx *ctx, int64_t *info);
  * cost less than 0.8 of the two products). Same operator, different rounding.
  * No counterpart in the reference. */
 int pqdts_set_hvp_path(pqdts_ctx *ctx, int path);
+
+/* ---- batched / served form (no counterpart in the reference) -------------
+ * A batch advances many independent problems together: every operator of the
+ * minimisation is one launch over all admitted problems (uneven M, N, D, gamma
+ * allowed), and one host synchronisation per step serves all of them. Each
+ * problem follows exactly the reference's algorithm on its own data (its own
+ * stages, CG exits and line searches); a problem's result does not depend on
+ * what else is in the batch. Contexts stay owned by the caller and must not be
+ * used through the single-context calls while admitted; call
+ * pqdts_set_hvp_path before admitting. */
+typedef struct pqdts_batch pqdts_batch;
+int pqdts_batch_create(int capacity, pqdts_batch **out);
+int pqdts_batch_destroy(pqdts_batch *b);
+/* Admits ctx (its X is the starting point) into a free slot. precision: 0 = FP64
+ * throughout; 1 = mixed: the Hessian products inside CG in FP32 MFMA with
+ * per-chunk FP64 accumulation, everything else (objective, gradient, line
+ * search, stopping tests, projections) in FP64. */
+int pqdts_batch_admit(pqdts_batch *b, pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage,
+                      int output, int precision, pqdts_event_cb cb, void *user, int *slot);
+/* One Newton iteration (plus stage switches) of every unfinished problem.
+ * Returns the number of finished problems still admitted (their slots in
+ * finished[0..cap)), or -1. */
+int pqdts_batch_step(pqdts_batch *b, int *finished, int cap);
+/* stats[0..3] = finished, Newton iterations, PCG iterations, line-search trials */
+int pqdts_batch_result(pqdts_batch *b, int slot, double *conv, int *stats);
+int pqdts_batch_release(pqdts_batch *b, int slot);
+int pqdts_batch_active(pqdts_batch *b);
+int pqdts_batch_kernel_timing(pqdts_batch *b, int on);
+int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms);
 
 /* simplexproj_condat_ (condat_simplexproj.c:60) over many rows at once: rows of
  * length n from y (host, row-major) projected onto the simplex into x (host).

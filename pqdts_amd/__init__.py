@@ -4,7 +4,7 @@ quantum detector tomography (a drop-in for pc2/pqdts's pqdts_omp.x).
 The compute path is libpqdts_amd.so (HIP kernels for gfx950 behind the C ABI
 in include/pqdts_amd.h). Python is a thin ctypes layer over it.
 """
-from ._lib import (COL_MAJOR, ROW_MAJOR, SLOTS, Context, PqdtsError, device_count, load,  # noqa: F401
-                   simplexproj_rows)
+from ._lib import (COL_MAJOR, ROW_MAJOR, SLOTS, Batch, Context, PqdtsError, device_count,  # noqa: F401
+                   load, simplexproj_rows)
 
-__all__ = ["Context", "PqdtsError", "device_count", "load", "simplexproj_rows", "SLOTS", "ROW_MAJOR", "COL_MAJOR"]
+__all__ = ["Batch", "Context", "PqdtsError", "device_count", "load", "simplexproj_rows", "SLOTS", "ROW_MAJOR", "COL_MAJOR"]

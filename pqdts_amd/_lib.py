@@ -27,6 +27,8 @@ EXPORTS = [
     "pqdts_pcopy", "pqdts_pscale", "pqdts_pcopy_scale", "pqdts_transform", "pqdts_lalpha_min2",
     "pqdts_lalpha_min_auglag", "pqdts_solve", "pqdts_timers", "pqdts_simplexproj_rows",
     "pqdts_kernel_timing", "pqdts_kernel_stats", "pqdts_info", "pqdts_set_hvp_path",
+    "pqdts_batch_create", "pqdts_batch_destroy", "pqdts_batch_admit", "pqdts_batch_step", "pqdts_batch_result",
+    "pqdts_batch_release", "pqdts_batch_active", "pqdts_batch_kernel_timing", "pqdts_batch_kernel_stats",
 ]
 
 _lib = None
@@ -82,6 +84,15 @@ def load():
     L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
     L.pqdts_set_hvp_path.argtypes = [vp, i32]
+    L.pqdts_batch_create.argtypes = [i32, C.POINTER(vp)]
+    L.pqdts_batch_destroy.argtypes = [vp]
+    L.pqdts_batch_admit.argtypes = [vp, vp, i32, i32, d, i32, i32, i32, EVENT_CB, vp, C.POINTER(i32)]
+    L.pqdts_batch_step.argtypes = [vp, C.POINTER(i32), i32]
+    L.pqdts_batch_result.argtypes = [vp, i32, dp, C.POINTER(i32)]
+    L.pqdts_batch_release.argtypes = [vp, i32]
+    L.pqdts_batch_active.argtypes = [vp]
+    L.pqdts_batch_kernel_timing.argtypes = [vp, i32]
+    L.pqdts_batch_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
     _lib = L
     return L
 
@@ -305,3 +316,92 @@ class Context:
         secs = (C.c_double * n)()
         L.pqdts_timers(self.h, n, C.cast(names, C.c_void_p), counts, secs)
         return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], secs[i]) for i in range(n)}
+
+
+def _kstats(fn, h):
+    n = fn(h, 0, None, None, None)
+    if n <= 0:
+        return {}
+    names = (C.c_char * 24 * n)()
+    counts = (C.c_int * n)()
+    ms = (C.c_double * n)()
+    fn(h, n, C.cast(names, C.c_void_p), counts, ms)
+    return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], ms[i]) for i in range(n)}
+
+
+class Batch:
+    """Many problems advanced together (pqdts_batch_*): one launch per operator
+    over all admitted problems, one host synchronisation per step."""
+
+    def __init__(self, capacity: int):
+        h = C.c_void_p()
+        _check(load().pqdts_batch_create(int(capacity), C.byref(h)))
+        self.h = h
+        self.capacity = int(capacity)
+        self.slots = {}   # slot -> (ctx, events list, callback keep-alive)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().pqdts_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, ctx: "Context", mode=2, maxiter=200, eps=1e-6, start_stage=1, output=0, precision=0,
+              record=True) -> int:
+        events = []
+
+        def cb(_u, kind, v, nv):
+            if record:
+                events.append((kind, tuple(v[i] for i in range(nv))))
+
+        fn = EVENT_CB(cb)
+        slot = C.c_int()
+        _check(load().pqdts_batch_admit(self.h, ctx.h, int(mode), int(maxiter), float(eps), int(start_stage),
+                                        int(output), int(precision), fn, None, C.byref(slot)))
+        self.slots[slot.value] = (ctx, events, fn)
+        return slot.value
+
+    def step(self):
+        """One Newton iteration of every unfinished problem; returns finished slots."""
+        buf = (C.c_int * max(1, self.capacity))()
+        n = load().pqdts_batch_step(self.h, buf, self.capacity)
+        if n < 0:
+            _check(-1)
+        return [buf[i] for i in range(min(n, self.capacity))]
+
+    def result(self, slot):
+        conv = C.c_double()
+        stats = (C.c_int * 4)()
+        _check(load().pqdts_batch_result(self.h, int(slot), C.byref(conv), stats))
+        return {"conv": conv.value, "finished": bool(stats[0]), "newton": stats[1], "pcg": stats[2],
+                "trials": stats[3]}
+
+    def events(self, slot):
+        return self.slots[slot][1]
+
+    def release(self, slot):
+        _check(load().pqdts_batch_release(self.h, int(slot)))
+        return self.slots.pop(slot, None)
+
+    def active(self) -> int:
+        return int(load().pqdts_batch_active(self.h))
+
+    def kernel_timing(self, on: bool):
+        _check(load().pqdts_batch_kernel_timing(self.h, int(bool(on))))
+
+    def kernel_stats(self) -> dict:
+        return _kstats(load().pqdts_batch_kernel_stats, self.h)
+
+    def run(self):
+        """Steps until every admitted problem has finished; returns {slot: result}."""
+        done = {}
+        while len(done) < len(self.slots):
+            for s in self.step():
+                if s not in done:
+                    done[s] = self.result(s)
+        return done

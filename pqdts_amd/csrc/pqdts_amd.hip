@@ -1,6 +1,12 @@
-// Host side of pqdts_amd: problem context, the C ABI declared in
-// include/pqdts_amd.h, and the minimisation loop (pqdts.f90:1908-2167) driving
-// the kernels of kernels.hip on one HIP stream.
+// Host side of pqdts_amd: problem contexts, the batched launch engine, the
+// lockstep minimisation loop (pqdts.f90:1908-2167) and the C ABI declared in
+// include/pqdts_amd.h.
+//
+// A context (pqdts_ctx) owns one problem's device arrays and an immutable device
+// descriptor. An Engine drives any number of contexts on one HIP stream: every
+// operator is one launch (per precision / Hessian path) over all the problems
+// that need it, and every host synchronisation fetches the scalars of all of
+// them at once. The plain solve is an engine with one problem.
 #include "../../include/pqdts_amd.h"
 #include "kernels.hip"
 
@@ -10,8 +16,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -51,56 +59,46 @@ int guarded(F &&f) {
   }
 }
 
-// tile heights of the two products
 constexpr int TILE = 64;  // rows and columns of a band-GEMM block tile
 constexpr int KC = 16;    // band chunk staged per LDS buffer (4 MFMA k-steps)
 
-// device scalar slots
-enum {
-  S_LA = 0, S_LB, S_XMIN, S_NACT, S_ND, S_B1, S_V2, S_RZ0, S_RZ1, S_RR, S_CONV, S_DOT, S_NSCAL = 16
-};
-
-// partial-sum regions (each PCAP doubles)
-enum { R_A = 0, R_B, R_C, R_D, R_NREG };
+// device scalar slots per problem
+enum { S_LA = 0, S_LB, S_XMIN, S_NACT, S_ND, S_B1, S_V2, S_RZ0, S_RZ1, S_RR, S_CONV, S_DOT };
+// partial-sum regions per problem
+enum { R_A = 0, R_B = 1, R_C = 2 };
 
 inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+template <class T>
+T *dalloc(int64_t n) {
+  void *p = nullptr;
+  HIPC(hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+  return (T *)p;
+}
 
 struct Timer {
   int count = 0;
   double sec = 0.0;
   std::chrono::steady_clock::time_point t0;
-  bool running = false;
 };
+
+struct Engine;
 
 }  // namespace
 
+// ===========================================================================
+// problem context
+// ===========================================================================
 struct pqdts_ctx {
   int M = 0, N = 0, D = 0;
   int64_t MN = 0, DN = 0;
-  int64_t nspan = 0;       // stored F elements: sum of row span lengths
-  int64_t tile_elems = 0;  // packed elements of F's band tiles (the MFMA work of O = F V)
   double gamma = 0.0;
-  int device = 0;
   hipStream_t s = nullptr;
-  // F as dense row spans
+  // device storage
   int *flo = nullptr, *flen = nullptr;
   int64_t *foff = nullptr;
   double *fval = nullptr;
   uint8_t *fhas = nullptr;
-  // F (tiles of D rows) and F^T (tiles of M rows) packed as dense transposed bands
-  struct Band {
-    int ntiles = 0;
-    double *pack = nullptr;
-    int64_t *off = nullptr;
-    int *klo = nullptr, *nch = nullptr;
-  } bf, bt, bg;  // bg: the Gram matrix G = F^T F (M x M), built on demand
-  int nrt = 0, nct = 0;
-  // HVP path: the two products through O = F p (F path) or one product with G
-  int hvp_pref = 0;           // 0 auto (cost model), 1 F path, 2 Gram path
-  bool gram_auto = false;     // what the cost model chose at create
-  int64_t gram_elems = 0;     // packed elements of G's band tiles (0 until known)
-  std::vector<int> glo, gw;   // G's row bands on the host (start, width)
-  bool use_gram() const { return hvp_pref == 2 || (hvp_pref == 0 && gram_auto); }
   int *kd_lo = nullptr, *kd_hi = nullptr;
   double *P = nullptr, *O = nullptr;
   double *slot[6] = {};
@@ -108,19 +106,100 @@ struct pqdts_ctx {
   int *maxr = nullptr;
   uint8_t *ik = nullptr;
   double *ev = nullptr, *gm = nullptr, *lam = nullptr, *lmin = nullptr;
-  double *part = nullptr;
+  double *part = nullptr, *scal = nullptr;
   int pcap = 0;
-  double *scal = nullptr, *hscal = nullptr;
-  bool full = true;
-  bool maxr_valid = false;  // every maxr[i] in [0, N)
-  int mode = 2;
-  bool sync_timers = false;
+  struct DBand {
+    int ntiles = 0;
+    double *pack = nullptr;
+    float *pack32 = nullptr;
+    int64_t *off = nullptr;
+    int *klo = nullptr, *nch = nullptr;
+    int64_t elems = 0;
+  } bf, bt, bg;
+  Prob desc{};               // host copy of the device descriptor
+  Prob *ddesc = nullptr;     // device descriptor (standalone use)
+  int64_t nspan = 0;
+  // state shared with the reference's module variables
+  bool full = true;          // tfullspace (pqdts.f90:97)
+  bool maxr_valid = false;   // every maxr[i] in [0, N)
+  // Hessian path and precision
+  int hvp_pref = 0;          // 0 auto, 1 F products, 2 Gram
+  bool gram_auto = false;
+  int64_t gram_elems = 0, gram_nnz = 0;
+  std::vector<int> glo, gw;
+  bool use_gram() const { return hvp_pref == 2 || (hvp_pref == 0 && gram_auto); }
+  std::unique_ptr<Engine> eng;  // standalone engine (this problem alone, on stream s)
+  Engine &engine();
+};
+
+namespace {
+
+Band to_band(const pqdts_ctx::DBand &b) {
+  Band r{};
+  r.pack = b.pack;
+  r.pack32 = b.pack32;
+  r.off = b.off;
+  r.klo = b.klo;
+  r.nch = b.nch;
+  r.ntiles = b.ntiles;
+  return r;
+}
+
+void refresh_desc(pqdts_ctx *c) {
+  Prob &p = c->desc;
+  p.M = c->M;
+  p.N = c->N;
+  p.D = c->D;
+  p.pcap = c->pcap;
+  p.gamma = c->gamma;
+  for (int i = 0; i < 6; i++) p.op[i] = c->slot[i];
+  p.op[OP_O] = c->O;
+  p.P = c->P;
+  p.diag_full = c->diag_full;
+  p.diag_red = c->diag_red;
+  p.maxr = c->maxr;
+  p.ik = c->ik;
+  p.ev = c->ev;
+  p.gm = c->gm;
+  p.lam = c->lam;
+  p.lmin = c->lmin;
+  p.flo = c->flo;
+  p.flen = c->flen;
+  p.foff = c->foff;
+  p.fval = c->fval;
+  p.fhas = c->fhas;
+  p.kdlo = c->kd_lo;
+  p.kdhi = c->kd_hi;
+  p.bf = to_band(c->bf);
+  p.bt = to_band(c->bt);
+  p.bg = to_band(c->bg);
+  p.part = c->part;
+  p.scal = c->scal;
+  HIPC(hipMemcpyAsync(c->ddesc, &c->desc, sizeof(Prob), hipMemcpyHostToDevice, c->s));
+  HIPC(hipStreamSynchronize(c->s));
+}
+
+// ===========================================================================
+// Engine: batched launches over problem slots
+// ===========================================================================
+struct PState;
+
+struct Engine {
+  hipStream_t s = nullptr;
+  bool own_stream = false;
+  int cap = 0;
+  std::vector<pqdts_ctx *> ctx;   // slot -> problem (nullptr: free)
+  Prob *dprobs = nullptr;         // device descriptors [cap]
+  double *dscal = nullptr;        // device scalars [cap * NSCAL] (batched engines)
+  double *hscal = nullptr;        // pinned host copy
+  bool private_scal = false;      // standalone: scalars live in the context
+  std::vector<double> arg;        // per-slot scalar argument of the next launch
+  std::vector<int> fp32;          // per-slot: mixed precision Hessian products
+  // timers
   std::map<std::string, Timer> timers;
   std::vector<std::string> timer_order;
-  pqdts_event_cb cb = nullptr;
-  void *user = nullptr;
-  // per-kernel HIP-event timing (pqdts_kernel_timing): pending start/stop pairs
-  // on the context's stream, folded into totals at every host synchronisation
+  bool sync_timers = false;
+  // per-kernel HIP-event timing
   bool ktiming = false;
   struct KPair {
     hipEvent_t a, b;
@@ -131,8 +210,60 @@ struct pqdts_ctx {
   std::vector<std::string> knames;
   std::vector<int> kcount;
   std::vector<double> kms;
+  std::vector<std::unique_ptr<PState>> st;  // solver state per slot
 
-  int kernel_id(const char *name) {
+  ~Engine();
+
+  void init(int capacity, hipStream_t stream, bool make_stream) {
+    cap = capacity;
+    ctx.assign(cap, nullptr);
+    arg.assign(cap, 0.0);
+    fp32.assign(cap, 0);
+    st.resize(cap);
+    if (make_stream) {
+      HIPC(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      own_stream = true;
+    } else {
+      s = stream;
+    }
+    dprobs = dalloc<Prob>(cap);
+    HIPC(hipHostMalloc((void **)&hscal, sizeof(double) * NSCAL * cap, hipHostMallocDefault));
+    sync_timers = getenv("PQDTS_SYNC_TIMERS") && atoi(getenv("PQDTS_SYNC_TIMERS")) != 0;
+  }
+
+  // Places a context in a slot; a batched engine redirects its scalars into the
+  // engine's contiguous array so that one copy fetches every problem's scalars.
+  void attach(int slot, pqdts_ctx *c) {
+    ctx[slot] = c;
+    Prob d = c->desc;
+    if (!private_scal) {
+      if (!dscal) dscal = dalloc<double>((int64_t)NSCAL * cap);
+      d.scal = dscal + (int64_t)NSCAL * slot;
+    }
+    HIPC(hipMemcpyAsync(dprobs + slot, &d, sizeof(Prob), hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+  }
+  void detach(int slot) {
+    ctx[slot] = nullptr;
+    st[slot].reset();
+  }
+
+  // ---- timing
+  void tstart(const char *name) {
+    auto it = timers.find(name);
+    if (it == timers.end()) {
+      it = timers.emplace(name, Timer{}).first;
+      timer_order.push_back(name);
+    }
+    it->second.count++;
+    it->second.t0 = std::chrono::steady_clock::now();
+  }
+  void tstop(const char *name) {
+    if (sync_timers) (void)hipStreamSynchronize(s);
+    auto &t = timers[name];
+    t.sec += std::chrono::duration<double>(std::chrono::steady_clock::now() - t.t0).count();
+  }
+  int kernel_id(const std::string &name) {
     for (size_t i = 0; i < knames.size(); i++)
       if (knames[i] == name) return (int)i;
     knames.push_back(name);
@@ -150,20 +281,18 @@ struct pqdts_ctx {
     HIPC(hipEventCreate(&e));
     return e;
   }
-  // returns the start event recorded before a launch (or nullptr)
   hipEvent_t kbegin() {
     if (!ktiming) return nullptr;
     hipEvent_t a = kevent();
     HIPC(hipEventRecord(a, s));
     return a;
   }
-  void kend(hipEvent_t a, const char *name) {
+  void kend(hipEvent_t a, const std::string &name) {
     if (!a) return;
     hipEvent_t b = kevent();
     HIPC(hipEventRecord(b, s));
     kpending.push_back({a, b, kernel_id(name)});
   }
-  // called after the stream has been synchronised
   void kfold() {
     for (auto &p : kpending) {
       float ms = 0.f;
@@ -176,503 +305,504 @@ struct pqdts_ctx {
     kpending.clear();
   }
 
-  double *reg(int r) { return part + (int64_t)r * pcap; }
-  const double *diag() const { return full ? diag_full : diag_red; }
-  int row_blocks() const { return cdiv(M, WAVES_PER_BLOCK); }
-
-  void tstart(const char *name) {
-    auto it = timers.find(name);
-    if (it == timers.end()) {
-      it = timers.emplace(name, Timer{}).first;
-      timer_order.push_back(name);
-    }
-    it->second.count++;
-    it->second.running = true;
-    it->second.t0 = std::chrono::steady_clock::now();
-  }
-  void tstop(const char *name) {
-    if (sync_timers) (void)hipStreamSynchronize(s);
-    auto &t = timers[name];
-    t.sec += std::chrono::duration<double>(std::chrono::steady_clock::now() - t.t0).count();
-    t.running = false;
-  }
-  void emit(int kind, std::initializer_list<double> v) {
-    if (cb) {
-      std::vector<double> tmp(v);
-      cb(user, kind, tmp.data(), (int)tmp.size());
-    }
-  }
-  // copy all device scalars to the host and wait for the stream
+  // ---- scalars
+  double *host_scal(int slot) { return hscal + (int64_t)NSCAL * slot; }
   void fetch() {
-    HIPC(hipMemcpyAsync(hscal, scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, s));
+    if (private_scal) {
+      HIPC(hipMemcpyAsync(hscal, ctx[0]->scal, sizeof(double) * NSCAL, hipMemcpyDeviceToHost, s));
+    } else {
+      HIPC(hipMemcpyAsync(hscal, dscal, sizeof(double) * NSCAL * cap, hipMemcpyDeviceToHost, s));
+    }
     HIPC(hipStreamSynchronize(s));
     if (!kpending.empty()) kfold();
   }
-  void reduce(int region, int n, int dst, bool is_min = false) {
-    k_reduce<<<1, BLOCK, 0, s>>>(reg(region), n, scal, dst, is_min ? 1 : 0);
-  }
-};
 
-namespace {
-
-struct Scope {
-  pqdts_ctx *c;
-  const char *n;
-  Scope(pqdts_ctx *c_, const char *n_) : c(c_), n(n_) { c->tstart(n); }
-  ~Scope() { c->tstop(n); }
-};
-
-double *slot_ptr(pqdts_ctx *c, int slot) {
-  require(slot >= 0 && slot < 6, "invalid slot id " + std::to_string(slot));
-  return c->slot[slot];
-}
-
-template <class T>
-T *dalloc(int64_t n) {
-  void *p = nullptr;
-  HIPC(hipMalloc(&p, sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
-  return (T *)p;
-}
-
-// ---------------------------------------------------------------------------
-// operators
-// ---------------------------------------------------------------------------
-
-// Reduced-space kernels index rows by maxr; refuse to launch them otherwise.
-void need_maxr(pqdts_ctx *c) {
-  require(c->maxr_valid, "reduced space needs a valid maxr (call pqdts_transform or pqdts_set_maxr)");
-}
-
-void op_reconstruct(pqdts_ctx *c, double *X) {
-  need_maxr(c);
-  k_reconstruct<<<c->row_blocks(), BLOCK, 0, c->s>>>(X, c->M, c->N, c->maxr);
-}
-
-BandArgs band_args(const pqdts_ctx::Band &b) {
-  BandArgs a{};
-  a.apack = b.pack;
-  a.aoff = b.off;
-  a.klo = b.klo;
-  a.kchunks = b.nch;
-  return a;
-}
-
-// O = F V (redO, pqdts.f90:724-736); returns the number of LOSS partials in R_A
-int op_spmm(pqdts_ctx *c, const double *V, bool expand, bool loss, bool store) {
-  BandArgs a = band_args(c->bf);
-  a.nrows = c->D;
-  a.nk = c->M;
-  a.N = c->N;
-  a.B = V;
-  a.maxr = c->maxr;
-  a.ev = c->ev;
-  a.C = c->O;
-  a.P = c->P;
-  a.has = c->fhas;
-  a.partial = c->reg(R_A);
-  dim3 grid(cdiv(c->N, TILE), c->bf.ntiles);
-  hipEvent_t ev0 = c->kbegin();
-  if (expand)
-    k_band_gemm<TILE, TILE, KC, B_EXPAND, E_STORE, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
-  else if (loss && store)
-    k_band_gemm<TILE, TILE, KC, B_PLAIN, E_STORE_LOSS, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
-  else if (loss)
-    k_band_gemm<TILE, TILE, KC, B_PLAIN, E_LOSS, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
-  else
-    k_band_gemm<TILE, TILE, KC, B_PLAIN, E_STORE, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
-  HIPC(hipGetLastError());
-  c->kend(ev0, "spmm_F");
-  return grid.x * grid.y;
-}
-
-// G = 2 F^T W + stencil (+ fold / mask / dot) (dL 998-1030, BF 276-340);
-// returns the number of DOT partials in R_B
-int op_spmmT(pqdts_ctx *c, int mode, const double *E, const double *dd, double *G, bool mask, bool dot) {
-  const bool red = !c->full;
-  if (red) need_maxr(c);
-  if (red) {
-    if (mode == T_GRAD)
-      k_foldval<T_GRAD><<<c->row_blocks(), BLOCK, 0, c->s>>>(c->O, c->P, c->N, c->M, c->flo, c->flen, c->foff,
-                                                            c->fval, c->kd_lo, c->kd_hi, c->gamma, E, c->maxr,
-                                                            c->ev, c->gm);
-    else
-      k_foldval<T_HVP><<<c->row_blocks(), BLOCK, 0, c->s>>>(c->O, c->P, c->N, c->M, c->flo, c->flen, c->foff,
-                                                           c->fval, c->kd_lo, c->kd_hi, c->gamma, E, c->maxr,
-                                                           c->ev, c->gm);
-  }
-  BandArgs a = band_args(c->bt);
-  a.nrows = c->M;
-  a.nk = c->D;
-  a.N = c->N;
-  a.B = c->O;
-  a.B2 = c->P;
-  a.maxr = c->maxr;
-  a.ev = c->ev;
-  a.C = G;
-  a.gamma = c->gamma;
-  a.E = E;
-  a.gm = c->gm;
-  a.ik = c->ik;
-  a.dd = dd;
-  a.partial = c->reg(R_B);
-  dim3 grid(cdiv(c->N, TILE), c->bt.ntiles);
-  hipEvent_t ev0 = c->kbegin();
-#define BG(BM, EM, R, MK, DT) k_band_gemm<TILE, TILE, KC, BM, EM, R, MK, DT><<<grid, BLOCK, 0, c->s>>>(a)
-  if (mode == T_GRAD) {
-    if (red)
-      BG(B_DIFF, E_GRAD, true, false, false);
-    else
-      BG(B_DIFF, E_GRAD, false, false, false);
-  } else {
-    if (red) {
-      if (mask && dot)
-        BG(B_PLAIN, E_HVP, true, true, true);
-      else
-        BG(B_PLAIN, E_HVP, true, false, false);
-    } else {
-      if (mask && dot)
-        BG(B_PLAIN, E_HVP, false, true, true);
-      else
-        BG(B_PLAIN, E_HVP, false, false, false);
-    }
-  }
-#undef BG
-  HIPC(hipGetLastError());
-  c->kend(ev0, "spmmT_F");
-  return grid.x * grid.y;
-}
-
-// objective (pqdts.f90:917-973); `recon` = apply the reduced-space prep
-double op_L(pqdts_ctx *c, double *X, bool recon = true, bool fetch = true) {
-  Scope t(c, "L");
-  if (!c->full && recon) op_reconstruct(c, X);
-  const int na = op_spmm(c, X, false, true, false);
-  c->reduce(R_A, na, S_LA);
-  if (c->gamma != 0.0) {
-    const int nb = std::min(cdiv((int64_t)(c->M - 1) * c->N, BLOCK), c->pcap);
-    if (nb > 0) {
-      k_smooth<<<nb, BLOCK, 0, c->s>>>(X, c->M, c->N, c->gamma, c->reg(R_B));
-      c->reduce(R_B, nb, S_LB);
-    } else {
-      HIPC(hipMemsetAsync(c->scal + S_LB, 0, sizeof(double), c->s));
-    }
-  } else {
-    HIPC(hipMemsetAsync(c->scal + S_LB, 0, sizeof(double), c->s));
-  }
-  if (!fetch) return 0.0;
-  c->fetch();
-  return c->hscal[S_LA] + c->hscal[S_LB];
-}
-
-// gradient (pqdts.f90:975-1032)
-void op_dL(pqdts_ctx *c, double *X, double *G) {
-  Scope t(c, "dL");
-  if (!c->full) op_reconstruct(c, X);
-  op_spmm(c, X, false, false, true);
-  op_spmmT(c, T_GRAD, X, nullptr, G, false, false);
-}
-
-// Hessian times dd (pqdts.f90:246-342). `have_ev`: ev already holds the
-// expansion of dd (computed by the CG kernels).
-void ensure_gram(pqdts_ctx *c);
-
-// Gram path of BF: BX = 2 G p~ + stencil (+ fold / mask / dot) with G = F^T F,
-// mathematically the same operator as 2 F^T (F p~) + stencil; no O is formed.
-int op_BF_gram(pqdts_ctx *c, const double *dd, double *BX, bool mask, bool dot) {
-  const bool red = !c->full;
-  if (red)
-    k_foldval_gram<<<c->row_blocks(), BLOCK, 0, c->s>>>(c->bg.pack, c->bg.off, c->bg.klo, c->bg.nch, KC, TILE,
-                                                         c->N, c->M, c->gamma, dd, c->maxr, c->ev, c->gm);
-  BandArgs a = band_args(c->bg);
-  a.nrows = c->M;
-  a.nk = c->M;
-  a.N = c->N;
-  a.B = dd;
-  a.maxr = c->maxr;
-  a.ev = c->ev;
-  a.C = BX;
-  a.gamma = c->gamma;
-  a.E = dd;
-  a.gm = c->gm;
-  a.ik = c->ik;
-  a.dd = dd;
-  a.partial = c->reg(R_B);
-  dim3 grid(cdiv(c->N, TILE), c->bg.ntiles);
-  hipEvent_t ev0 = c->kbegin();
-#define BG(BM, R, MK, DT) k_band_gemm<TILE, TILE, KC, BM, E_HVP, R, MK, DT><<<grid, BLOCK, 0, c->s>>>(a)
-  if (red) {
-    if (mask && dot)
-      BG(B_EXPAND, true, true, true);
-    else
-      BG(B_EXPAND, true, false, false);
-  } else {
-    if (mask && dot)
-      BG(B_PLAIN, false, true, true);
-    else
-      BG(B_PLAIN, false, false, false);
-  }
-#undef BG
-  HIPC(hipGetLastError());
-  c->kend(ev0, "hvp_G");
-  return grid.x * grid.y;
-}
-
-int op_BF(pqdts_ctx *c, const double *dd, double *BX, bool have_ev, bool mask, bool dot) {
-  Scope t(c, "BF");
-  if (!c->full) need_maxr(c);
-  if (!c->full && !have_ev)
-    k_others<<<c->row_blocks(), BLOCK, 0, c->s>>>(dd, c->M, c->N, c->maxr, c->ev);
-  if (c->use_gram()) {
-    ensure_gram(c);
-    return op_BF_gram(c, dd, BX, mask, dot);
-  }
-  op_spmm(c, dd, !c->full, false, true);
-  return op_spmmT(c, T_HVP, dd, dd, BX, mask, dot);
-}
-
-double op_pdot(pqdts_ctx *c, const double *x, const double *y) {
-  Scope t(c, "pdot");
-  const int nb = std::min(cdiv(c->MN, BLOCK), std::min(c->pcap, 1024));
-  k_dot<<<nb, BLOCK, 0, c->s>>>(x, y, c->MN, c->reg(R_A));
-  c->reduce(R_A, nb, S_DOT);
-  c->fetch();
-  return c->hscal[S_DOT];
-}
-
-void op_doproj(pqdts_ctx *c, double *X) {
-  Scope t(c, "doproj");
-  k_simplex_rows<<<c->row_blocks(), BLOCK, 0, c->s>>>(X, nullptr, nullptr, 0.0, c->M, c->N, 0);
-  HIPC(hipGetLastError());
-}
-
-double op_doproj2(pqdts_ctx *c, double *X, bool fetch = true) {
-  Scope t(c, "doproj2");
-  require(!c->full, "doproj2 needs the reduced space (maxr set)");
-  need_maxr(c);
-  const int nb = c->row_blocks();
-  k_clip_rows<<<nb, BLOCK, 0, c->s>>>(X, nullptr, nullptr, 0.0, c->M, c->N, c->maxr, 0, 0, c->reg(R_C));
-  c->reduce(R_C, nb, S_XMIN, true);
-  if (!fetch) return 0.0;
-  c->fetch();
-  return c->hscal[S_XMIN];
-}
-
-// conv_test (pqdts.f90:1034-1067)
-double op_conv(pqdts_ctx *c, double *X, double *G, bool print) {
-  if (!c->full) op_reconstruct(c, X);
-  c->full = true;
-  op_dL(c, X, G);
-  const int nb = c->row_blocks();
-  k_conv_rows<<<nb, BLOCK, 0, c->s>>>(X, G, c->M, c->N, print ? c->lam : nullptr, print ? c->lmin : nullptr,
-                                      c->reg(R_C));
-  c->reduce(R_C, nb, S_CONV);
-  c->fetch();
-  if (print && c->cb) {
-    std::vector<double> lam(c->M), lmin(c->M);
-    HIPC(hipMemcpy(lam.data(), c->lam, sizeof(double) * c->M, hipMemcpyDeviceToHost));
-    HIPC(hipMemcpy(lmin.data(), c->lmin, sizeof(double) * c->M, hipMemcpyDeviceToHost));
-    for (int i = 0; i < c->M; i++) c->emit(PQDTS_EV_LAMBDA, {(double)(i + 1), lam[i], lmin[i]});
-  }
-  return std::sqrt(c->hscal[S_CONV] / ((double)c->N * (double)c->M));
-}
-
-// Objective at the trial point X + a*SN after projection (both line searches).
-// Returns L; *xmin receives the doproj2 minimum (0 for Condat).
-double trial_L(pqdts_ctx *c, double a, bool condat, double *xmin) {
-  double *T = c->slot[PQDTS_XTMP];
-  const int nb = c->row_blocks();
-  if (condat) {
-    Scope t(c, "doproj");
-    k_simplex_rows<<<nb, BLOCK, 0, c->s>>>(T, c->slot[PQDTS_X], c->slot[PQDTS_SN], a, c->M, c->N, 1);
-  } else {
-    Scope t(c, "doproj2");
-    k_clip_rows<<<nb, BLOCK, 0, c->s>>>(T, c->slot[PQDTS_X], c->slot[PQDTS_SN], a, c->M, c->N, c->maxr, 1, 1,
-                                        c->reg(R_C));
-    c->reduce(R_C, nb, S_XMIN, true);
-  }
-  HIPC(hipGetLastError());
-  // the reduced-space prep of L was applied inside k_clip_rows
-  double L = op_L(c, T, false, true);
-  *xmin = condat ? 0.0 : c->hscal[S_XMIN];
-  return L;
-}
-
-// Lalpha_min2 (pqdts.f90:394-438) and Lalpha_min_auglag (pqdts.f90:440-483)
-void op_linesearch(pqdts_ctx *c, bool condat, double *alpha_out, double *dlalpha) {
-  Scope t(c, "Lalpha_min");
-  if (!condat) {
-    require(!c->full, "Lalpha_min2 needs the reduced space (maxr set)");
-    need_maxr(c);
-  }
-  const double dx = (double)1e-5f;        // pqdts.f90:404
-  const double feas = (double)-1e-12f;    // pqdts.f90:424
-  double xmin;
-  const double L0 = trial_L(c, 0.0, condat, &xmin);
-  double Lp = trial_L(c, 0.0 + dx, condat, &xmin);
-  const double df = (Lp - L0) / dx;
-  *dlalpha = df;
-  double alpha = 1.0, lm = std::numeric_limits<double>::max(), alphamin = 0.0;
-  for (int i = 1; i <= 100; i++) {
-    Lp = trial_L(c, alpha, condat, &xmin);
-    const bool feasible = condat || xmin >= feas;
-    if (Lp < lm && feasible) {
-      lm = Lp;
-      alphamin = alpha;
-    }
-    c->emit(PQDTS_EV_LS, {(double)i, Lp, alpha, df, xmin, condat ? 0.0 : 1.0});
-    if (Lp > L0 + 0.1 * alpha * df || !feasible)
-      alpha = alpha * 0.75;
-    else
-      break;
-  }
-  *alpha_out = alphamin;
-}
-
-void op_axpby(pqdts_ctx *c, double *z, const double *x, const double *y, double alpha, double beta, int kind) {
-  const int nb = std::max(1, std::min(cdiv(c->MN, BLOCK), 4096));
-  k_axpby<<<nb, BLOCK, 0, c->s>>>(z, x, y, alpha, beta, c->MN, kind);
-  HIPC(hipGetLastError());
-}
-
-void op_transform(pqdts_ctx *c, bool rebuild) {
-  k_transform<<<c->row_blocks(), BLOCK, 0, c->s>>>(c->slot[PQDTS_X], c->M, c->N, c->maxr, rebuild ? 1 : 0);
-  HIPC(hipGetLastError());
-  c->maxr_valid = true;
-}
-
-void fill_maxr(pqdts_ctx *c, int v) {
-  std::vector<int> h(c->M, v);
-  HIPC(hipMemcpyAsync(c->maxr, h.data(), sizeof(int) * c->M, hipMemcpyHostToDevice, c->s));
-  HIPC(hipStreamSynchronize(c->s));
-  c->maxr_valid = v >= 0 && v < c->N;
-}
-
-// ---------------------------------------------------------------------------
-// The minimisation loop (pqdts.f90:1908-2167)
-// ---------------------------------------------------------------------------
-double solve(pqdts_ctx *c, int mode, int maxiter, double conv_thres, int start_stage, int output) {
-  Scope tmin(c, "min");
-  double *X = c->slot[PQDTS_X], *G = c->slot[PQDTS_G], *HP = c->slot[PQDTS_HP];
-  double *XT = c->slot[PQDTS_XTMP], *SN = c->slot[PQDTS_SN], *Z = c->slot[PQDTS_Z];
-  const int M = c->M, N = c->N;
-  const int64_t MN = c->MN;
-  c->mode = mode;
-  double conv = 0.0, alpha = 0.0, dlalpha, norm_der, L0, xmin;
-
-  L0 = op_L(c, X);
-  op_dL(c, X, G);
-  norm_der = op_pdot(c, G, G);
-  c->emit(PQDTS_EV_ITER0, {L0, std::sqrt(norm_der)});
-
-  for (int oiter = start_stage; oiter <= 2; oiter++) {
-    dlalpha = -std::numeric_limits<double>::max();
-    for (int iiter = 1; iiter <= maxiter; iiter++) {
-      if (mode == 1) {
-        // projected gradient, pqdts.f90:1934-1951
-        c->full = false;
-        op_transform(c, false);
-        op_dL(c, X, G);
-        op_axpby(c, SN, G, nullptr, -1.0, 0.0, 2);
-        op_linesearch(c, false, &alpha, &dlalpha);
-      } else {
-        if ((std::fabs(dlalpha) < (double)1e-4f || dlalpha > 0) && oiter == 1) {
-          c->emit(PQDTS_EV_STOP1, {dlalpha});
-          break;
-        }
-        c->full = (oiter == 1);
-        if (!c->full)
-          op_transform(c, true);
-        else
-          fill_maxr(c, -1);
-        op_dL(c, X, G);
-        const bool red = !c->full;
-        if (red) need_maxr(c);
-        const int nb = c->row_blocks();
-        {
-          Scope t(c, "active_dLdX");
-          k_cg_init<<<nb, BLOCK, 0, c->s>>>(X, G, c->ik, Z, XT, SN, c->diag(), M, N, c->maxr, c->ev, red ? 1 : 0,
-                                            1, c->reg(R_A), c->reg(R_B), c->reg(R_C));
-          HIPC(hipGetLastError());
-          c->reduce(R_A, nb, S_NACT);
-          c->reduce(R_B, nb, S_ND);
-          c->reduce(R_C, nb, S_B1);
-          c->fetch();
-        }
-        c->emit(PQDTS_EV_ACTIVE, {c->hscal[S_NACT], (double)MN});
-        norm_der = c->hscal[S_ND];
-        const double eps = std::fmin(0.5, std::sqrt(std::sqrt(norm_der))) * std::sqrt(norm_der);
-        int i_b1 = S_B1;
-        for (int j = 1; j <= 50; j++) {
-          const int ndot = op_BF(c, XT, HP, true, true, true);
-          Scope t(c, "CG");
-          c->reduce(R_B, ndot, S_V2);
-          const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
-          k_cg_update<<<nb, BLOCK, 0, c->s>>>(SN, XT, G, HP, Z, c->diag(), M, N, c->scal, i_b1, S_V2, c->reg(R_A),
-                                              c->reg(R_C));
-          HIPC(hipGetLastError());
-          c->reduce(R_A, nb, i_rz);
-          c->reduce(R_C, nb, S_RR);
-          k_cg_dir<<<nb, BLOCK, 0, c->s>>>(XT, Z, M, N, c->scal, i_b1, i_rz, c->maxr, c->ev, red ? 1 : 0);
-          HIPC(hipGetLastError());
-          c->fetch();
-          const double b1 = c->hscal[i_b1], v2 = c->hscal[i_rz], b2 = c->hscal[S_RR];
-          c->emit(PQDTS_EV_PCG, {(double)j, std::sqrt(b2), eps, b1, v2});
-          if (std::sqrt(b2) <= eps) break;
-          i_b1 = i_rz;
-        }
-        // two-metric diagonal step, pqdts.f90:2092-2104
-        op_dL(c, X, G);
-        k_twometric<<<nb, BLOCK, 0, c->s>>>(SN, G, c->ik, c->diag(), M, N, c->maxr);
-        HIPC(hipGetLastError());
-        op_linesearch(c, c->full, &alpha, &dlalpha);
+  // ---- work lists: groups of at most MAXB slots; nb(slot) blocks each
+  template <class NB, class L>
+  void launch(const std::vector<int> &S, NB nb, L fn) {
+    for (size_t g = 0; g < S.size(); g += MAXB) {
+      WorkList wl;
+      wl.np = (int)std::min<size_t>(MAXB, S.size() - g);
+      int tot = 0;
+      for (int i = 0; i < wl.np; i++) {
+        const int p = S[g + i];
+        wl.pid[i] = p;
+        wl.boff[i] = tot;
+        tot += nb(p);
+        wl.flags[i] = ctx[p]->full ? F_FULL : 0;
+        wl.a[i] = arg[p];
       }
-      // step, projection and re-evaluation, pqdts.f90:2111-2128
-      if (alpha != 0.0) op_axpby(c, X, SN, X, alpha, 0.0, 1);
-      xmin = 0.0;
-      if (c->full)
-        op_doproj(c, X);
-      else
-        xmin = op_doproj2(c, X);
-      L0 = op_L(c, X);
-      op_dL(c, X, G);
-      norm_der = op_pdot(c, G, G);
-      conv = op_conv(c, X, G, false);
-      c->emit(PQDTS_EV_ITER,
-              {(double)oiter, (double)iiter, L0, xmin, conv, std::sqrt(norm_der), (double)mode, alpha, dlalpha});
-      if (conv < conv_thres) break;
+      wl.boff[wl.np] = tot;
+      if (tot > 0) fn(wl, tot);
+      HIPC(hipGetLastError());
     }
-    conv = op_conv(c, X, G, output >= 3);
-    c->emit(PQDTS_EV_STAGE, {(double)oiter, conv});
   }
-  HIPC(hipStreamSynchronize(c->s));
-  c->kfold();
-  return conv;
-}
+  // a producer launch followed by the per-problem reduction of its partials
+  template <class NB, class L>
+  void launch_reduce(const std::vector<int> &S, NB nb, L fn, int region, int dst, bool is_min = false) {
+    launch(S, nb, [&](const WorkList &wl, int tot) {
+      fn(wl, tot);
+      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, region, dst, is_min ? 1 : 0);
+    });
+  }
 
-void destroy_ctx(pqdts_ctx *c) {
-  if (!c) return;
-  if (c->s) (void)hipStreamSynchronize(c->s);
-  for (auto &p : c->kpending) {
+  int nb_rows(int p) const { return cdiv(ctx[p]->M, WAVES_PER_BLOCK); }
+  int nb_elem(int p) const { return std::max(1, cdiv(ctx[p]->MN, EBLOCK)); }
+  int nb_gemm(int p, int band) const {
+    const pqdts_ctx *c = ctx[p];
+    const int nt = band == BAND_F ? c->bf.ntiles : band == BAND_FT ? c->bt.ntiles : c->bg.ntiles;
+    return cdiv(c->N, TILE) * nt;
+  }
+
+  std::vector<int> reduced_of(const std::vector<int> &S) const {
+    std::vector<int> r;
+    for (int p : S)
+      if (!ctx[p]->full) r.push_back(p);
+    return r;
+  }
+  void need_maxr(const std::vector<int> &S) const {
+    for (int p : S)
+      require(ctx[p]->full || ctx[p]->maxr_valid,
+              "reduced space needs a valid maxr (call pqdts_transform or pqdts_set_maxr)");
+  }
+
+  // ---- band GEMM dispatch
+  template <typename T, int BM, int EM, bool MASK, bool DOT>
+  void gemm_t(const std::vector<int> &S, GemmOps ops, const char *kname) {
+    hipEvent_t e0 = kbegin();
+    launch(S, [&](int p) { return nb_gemm(p, ops.band); }, [&](const WorkList &wl, int tot) {
+      kb_band_gemm<T, KC, BM, EM, MASK, DOT><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
+    });
+    kend(e0, kname);
+  }
+
+  // ================= operators (reference subroutine in brackets) =================
+
+  // X[j,maxr] = 1 - (sum - X[j,maxr]) for reduced problems (L/dL/conv prep)
+  void reconstruct(const std::vector<int> &S, int x) {
+    need_maxr(S);
+    launch(S, [&](int p) { return nb_rows(p); },
+           [&](const WorkList &wl, int tot) { kb_reconstruct<<<tot, BLOCK, 0, s>>>(dprobs, wl, x); });
+  }
+
+  // L (pqdts.f90:917-973): scalars S_LA (data term) and S_LB (smoothing)
+  void L(const std::vector<int> &S, int x, bool recon) {
+    struct Scope sc(this, "L");
+    if (recon) reconstruct(S, x);
+    GemmOps ops{BAND_F, x, OP_O, 0, 0, R_A};
+    hipEvent_t e0 = kbegin();
+    launch_reduce(S, [&](int p) { return nb_gemm(p, BAND_F); }, [&](const WorkList &wl, int tot) {
+      kb_band_gemm<double, KC, B_PLAIN, E_LOSS, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
+    }, R_A, S_LA);
+    kend(e0, "spmm_F");
+    launch_reduce(S, [&](int p) { return std::max(0, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); },
+                  [&](const WorkList &wl, int tot) { kb_smooth<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, R_B); }, R_B,
+                  S_LB);
+  }
+  double Lval(int p) { return host_scal(p)[S_LA] + host_scal(p)[S_LB]; }
+
+  // dL (pqdts.f90:975-1032)
+  void dL(const std::vector<int> &S, int x, int g) {
+    struct Scope sc(this, "dL");
+    reconstruct(S, x);
+    gemm_t<double, B_PLAIN, E_STORE, false, false>(S, GemmOps{BAND_F, x, OP_O, 0, 0, 0}, "spmm_F");
+    std::vector<int> R = reduced_of(S);
+    if (!R.empty())
+      launch(R, [&](int p) { return nb_rows(p); },
+             [&](const WorkList &wl, int tot) { kb_foldval<T_GRAD><<<tot, BLOCK, 0, s>>>(dprobs, wl, x); });
+    gemm_t<double, B_DIFF, E_GRAD, false, false>(S, GemmOps{BAND_FT, OP_O, g, x, 0, 0}, "spmmT_F");
+  }
+
+  // BF (pqdts.f90:246-342): bx = H dd; with `dot` the partial dd.bx goes to S_V2
+  void BF(const std::vector<int> &S, int dd, int bx, bool have_ev, bool mask, bool dot) {
+    struct Scope sc(this, "BF");
+    need_maxr(S);
+    std::vector<int> R = reduced_of(S);
+    if (!have_ev && !R.empty())
+      launch(R, [&](int p) { return nb_rows(p); },
+             [&](const WorkList &wl, int tot) { kb_others<<<tot, BLOCK, 0, s>>>(dprobs, wl, dd); });
+    std::vector<int> g64, g32, f64, f32;
+    for (int p : S) {
+      const bool gram = ctx[p]->use_gram();
+      (gram ? (fp32[p] ? g32 : g64) : (fp32[p] ? f32 : f64)).push_back(p);
+    }
+    // Gram path: fold values, then one product 2 G p~ with the epilogue
+    std::vector<int> G = g64;
+    G.insert(G.end(), g32.begin(), g32.end());
+    std::vector<int> GR = reduced_of(G);
+    if (!GR.empty())
+      launch(GR, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+        kb_foldval_gram<<<tot, BLOCK, 0, s>>>(dprobs, wl, dd, KC, TILE);
+      });
+    const GemmOps hg{BAND_G, dd, bx, dd, dd, R_B};
+    // B = p with its implicit columns expanded (reduced problems)
+    hvp_gemm<double, B_EXPAND>(g64, hg, mask, dot, "hvp_G");
+    hvp_gemm<float, B_EXPAND>(g32, hg, mask, dot, "hvp_G32");
+    // F path: O = F p~, fold values, then 2 F^T O with the epilogue
+    for (int pass = 0; pass < 2; pass++) {
+      const std::vector<int> &FS = pass ? f32 : f64;
+      if (FS.empty()) continue;
+      if (pass)
+        gemm_t<float, B_EXPAND, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F32");
+      else
+        gemm_t<double, B_EXPAND, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F");
+      std::vector<int> FR = reduced_of(FS);
+      if (!FR.empty())
+        launch(FR, [&](int p) { return nb_rows(p); },
+               [&](const WorkList &wl, int tot) { kb_foldval<T_HVP><<<tot, BLOCK, 0, s>>>(dprobs, wl, dd); });
+      const GemmOps ht{BAND_FT, OP_O, bx, dd, dd, R_B};
+      if (pass)
+        hvp_gemm<float, B_PLAIN>(FS, ht, mask, dot, "spmmT_F32");
+      else
+        hvp_gemm<double, B_PLAIN>(FS, ht, mask, dot, "spmmT_F");
+    }
+  }
+  template <typename T, int BM>
+  void hvp_gemm(const std::vector<int> &S, GemmOps ops, bool mask, bool dot, const char *kname) {
+    if (S.empty()) return;
+    hipEvent_t e0 = kbegin();
+    auto nb = [&](int p) { return nb_gemm(p, ops.band); };
+    if (mask && dot) {
+      launch_reduce(S, nb, [&](const WorkList &wl, int tot) {
+        kb_band_gemm<T, KC, BM, E_HVP, true, true><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
+      }, R_B, S_V2);
+    } else {
+      launch(S, nb, [&](const WorkList &wl, int tot) {
+        kb_band_gemm<T, KC, BM, E_HVP, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
+      });
+    }
+    kend(e0, kname);
+  }
+
+  // pdot (pqdts.f90:589-614) -> S_DOT
+  void pdot(const std::vector<int> &S, int x, int y) {
+    struct Scope sc(this, "pdot");
+    launch_reduce(S, [&](int p) { return nb_elem(p); },
+                  [&](const WorkList &wl, int tot) { kb_dot<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, y, R_A); }, R_A,
+                  S_DOT);
+  }
+
+  // doproj / doproj2 (pqdts.f90:1120-1164) on operand t; with form_trial the
+  // trial point t = arg*SN + X is formed first. xmin -> S_XMIN.
+  void project(const std::vector<int> &S, int t, bool form_trial, bool recon_again) {
+    need_maxr(S);
+    bool any_full = false, any_red = false;
+    for (int p : S) (ctx[p]->full ? any_full : any_red) = true;
+    struct Scope sc(this, any_red && !any_full ? "doproj2" : "doproj");
+    launch_reduce(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kb_project<<<tot, BLOCK, 0, s>>>(dprobs, wl, t, form_trial ? 1 : 0, recon_again ? 1 : 0, R_C);
+    }, R_C, S_XMIN, true);
+  }
+
+  // conv_test (pqdts.f90:1034-1067): sets full space; -> S_CONV
+  void conv(const std::vector<int> &S, int x, int g, bool store) {
+    reconstruct(S, x);
+    for (int p : S) ctx[p]->full = true;
+    dL(S, x, g);
+    launch_reduce(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kb_conv_rows<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, g, store ? 1 : 0, R_C);
+    }, R_C, S_CONV);
+  }
+  double convval(int p) {
+    return std::sqrt(host_scal(p)[S_CONV] / ((double)ctx[p]->N * (double)ctx[p]->M));
+  }
+
+  void axpby(const std::vector<int> &S, int z, int x, int y, double alpha, double beta, int kind,
+             bool per_problem_alpha) {
+    launch(S, [&](int p) { return nb_elem(p); }, [&](const WorkList &wl, int tot) {
+      kb_axpby<<<tot, BLOCK, 0, s>>>(dprobs, wl, z, x, y, alpha, beta, kind, per_problem_alpha ? 1 : 0);
+    });
+  }
+
+  void transform(const std::vector<int> &S, bool rebuild) {
+    launch(S, [&](int p) { return nb_rows(p); },
+           [&](const WorkList &wl, int tot) { kb_transform<<<tot, BLOCK, 0, s>>>(dprobs, wl, rebuild ? 1 : 0); });
+    for (int p : S) ctx[p]->maxr_valid = !ctx[p]->full;
+  }
+
+  // ---- Newton-CG pieces (pqdts.f90:1986-2104)
+  void cg_init(const std::vector<int> &S) {
+    struct Scope sc(this, "active_dLdX");
+    need_maxr(S);
+    launch(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kb_cg_init<<<tot, BLOCK, 0, s>>>(dprobs, wl, 1);
+      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_A, S_NACT, 0);
+      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_B, S_ND, 0);
+      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_C, S_B1, 0);
+    });
+  }
+  void cg_update(const std::vector<int> &S, int i_b1, int i_rz) {
+    launch(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kb_cg_update<<<tot, BLOCK, 0, s>>>(dprobs, wl, i_b1, S_V2);
+      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_A, i_rz, 0);
+      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_C, S_RR, 0);
+    });
+  }
+  void cg_dir(const std::vector<int> &S, int i_b1, int i_rz) {
+    launch(S, [&](int p) { return nb_rows(p); },
+           [&](const WorkList &wl, int tot) { kb_cg_dir<<<tot, BLOCK, 0, s>>>(dprobs, wl, i_b1, i_rz); });
+  }
+  void twometric(const std::vector<int> &S) {
+    launch(S, [&](int p) { return nb_rows(p); },
+           [&](const WorkList &wl, int tot) { kb_twometric<<<tot, BLOCK, 0, s>>>(dprobs, wl); });
+  }
+
+  struct Scope {
+    Engine *e;
+    const char *n;
+    Scope(Engine *e_, const char *n_) : e(e_), n(n_) { e->tstart(n); }
+    ~Scope() { e->tstop(n); }
+  };
+
+  // ---- solver (defined below)
+  void line_search(const std::vector<int> &A);
+  void tick();
+  bool any_active() const;
+};
+
+// Per-problem progress of the minimisation (the reference's loop variables).
+struct PState {
+  int mode = 2, maxiter = 0, start_stage = 1, output = 0;
+  double eps = 0.0;
+  pqdts_event_cb cb = nullptr;
+  void *user = nullptr;
+  int phase = 0;  // 0 initial evaluation pending, 1 iterating, 2 done
+  int stage = 1, iiter = 0;
+  bool stage_exit = false;
+  double dlalpha = 0.0, alpha = 0.0, conv = 0.0, eps_cg = 0.0;
+  int ib1 = S_B1;
+  // line search
+  double ls_L0 = 0.0, ls_df = 0.0, ls_alpha = 0.0, ls_lm = 0.0, ls_amin = 0.0;
+  // counters
+  int newton = 0, pcg = 0, trials = 0;
+  void emit(int kind, std::initializer_list<double> v) {
+    if (cb) {
+      std::vector<double> t(v);
+      cb(user, kind, t.data(), (int)t.size());
+    }
+  }
+};
+
+Engine::~Engine() {
+  if (s) (void)hipStreamSynchronize(s);
+  for (auto &p : kpending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
   }
-  for (hipEvent_t e : c->kpool) (void)hipEventDestroy(e);
-  void *ptrs[] = {c->flo,      c->flen,      c->foff,   c->fval,   c->fhas,   c->bf.pack, c->bf.off,
-                  c->bf.klo,   c->bf.nch,    c->bt.pack, c->bt.off, c->bt.klo, c->bt.nch,  c->bg.pack,
-                  c->bg.off,   c->bg.klo,    c->bg.nch,  c->kd_lo,
-                  c->kd_hi,    c->P,         c->O,      c->diag_full, c->diag_red, c->maxr, c->ik,
-                  c->ev,       c->gm,        c->lam,    c->lmin,   c->part,   c->scal};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
-  for (double *p : c->slot)
-    if (p) (void)hipFree(p);
-  if (c->hscal) (void)hipHostFree(c->hscal);
-  if (c->s) (void)hipStreamDestroy(c->s);
-  delete c;
+  for (hipEvent_t e : kpool) (void)hipEventDestroy(e);
+  if (dprobs) (void)hipFree(dprobs);
+  if (dscal) (void)hipFree(dscal);
+  if (hscal) (void)hipHostFree(hscal);
+  if (own_stream && s) (void)hipStreamDestroy(s);
 }
 
+bool Engine::any_active() const {
+  for (int p = 0; p < cap; p++)
+    if (ctx[p] && st[p] && st[p]->phase != 2) return true;
+  return false;
+}
+
+// Lalpha_min_auglag (stage 1, pqdts.f90:440-483) for full-space problems and
+// Lalpha_min2 (pqdts.f90:394-438) for reduced ones, in lockstep: every round
+// evaluates one trial point for each problem still searching.
+void Engine::line_search(const std::vector<int> &A) {
+  Scope sc(this, "Lalpha_min");
+  const double dx = (double)1e-5f;      // pqdts.f90:354/404/450
+  const double feas = (double)-1e-12f;  // pqdts.f90:424/430
+  auto trial = [&](const std::vector<int> &S) {
+    project(S, PQDTS_XTMP, true, true);
+    L(S, PQDTS_XTMP, false);  // the reduced-space prep was applied by kb_project
+    fetch();
+  };
+  for (int p : A) arg[p] = 0.0;
+  trial(A);
+  for (int p : A) st[p]->ls_L0 = Lval(p);
+  for (int p : A) arg[p] = 0.0 + dx;
+  trial(A);
+  for (int p : A) {
+    PState &q = *st[p];
+    q.ls_df = (Lval(p) - q.ls_L0) / dx;
+    q.dlalpha = q.ls_df;
+    q.ls_alpha = 1.0;
+    q.ls_lm = std::numeric_limits<double>::max();
+    q.ls_amin = 0.0;
+  }
+  std::vector<int> S = A;
+  for (int i = 1; i <= 100 && !S.empty(); i++) {
+    for (int p : S) arg[p] = st[p]->ls_alpha;
+    trial(S);
+    std::vector<int> next;
+    for (int p : S) {
+      PState &q = *st[p];
+      const bool min2 = !ctx[p]->full;
+      const double Lp = Lval(p), xmin = min2 ? host_scal(p)[S_XMIN] : 0.0;
+      const bool feasible = !min2 || xmin >= feas;
+      q.trials++;
+      if (Lp < q.ls_lm && feasible) {
+        q.ls_lm = Lp;
+        q.ls_amin = q.ls_alpha;
+      }
+      q.emit(PQDTS_EV_LS, {(double)i, Lp, q.ls_alpha, q.ls_df, xmin, min2 ? 1.0 : 0.0});
+      if (Lp > q.ls_L0 + 0.1 * q.ls_alpha * q.ls_df || !feasible) {
+        q.ls_alpha = q.ls_alpha * 0.75;
+        next.push_back(p);
+      }
+    }
+    S.swap(next);
+  }
+  for (int p : A) st[p]->alpha = st[p]->ls_amin;
+}
+
+// One Newton iteration (or the initial evaluation / stage switches) for every
+// problem of the engine, pqdts.f90:1908-2167.
+void Engine::tick() {
+  const int X = PQDTS_X, G = PQDTS_G, HP = PQDTS_HP, XT = PQDTS_XTMP, SN = PQDTS_SN;
+  std::vector<int> init;
+  for (int p = 0; p < cap; p++)
+    if (ctx[p] && st[p] && st[p]->phase == 0) init.push_back(p);
+  if (!init.empty()) {
+    // pqdts.f90:1908-1915 (full space, as set at program start)
+    L(init, X, true);
+    dL(init, X, G);
+    pdot(init, G, G);
+    fetch();
+    for (int p : init) {
+      PState &q = *st[p];
+      q.emit(PQDTS_EV_ITER0, {Lval(p), std::sqrt(host_scal(p)[S_DOT])});
+      q.phase = 1;
+      q.stage = q.start_stage;
+      q.iiter = 0;
+      q.dlalpha = -std::numeric_limits<double>::max();
+      q.stage_exit = false;
+    }
+  }
+  // stage control: exits at the top of an iteration and at maxiter
+  for (;;) {
+    std::vector<int> E, Elam;
+    for (int p = 0; p < cap; p++) {
+      if (!ctx[p] || !st[p] || st[p]->phase != 1) continue;
+      PState &q = *st[p];
+      bool end = q.stage_exit || q.iiter >= q.maxiter;
+      if (!end && q.mode == 2 && q.stage == 1 && (std::fabs(q.dlalpha) < (double)1e-4f || q.dlalpha > 0)) {
+        q.emit(PQDTS_EV_STOP1, {q.dlalpha});  // pqdts.f90:1955-1957
+        end = true;
+      }
+      if (end) (q.output >= 3 ? Elam : E).push_back(p);
+    }
+    if (E.empty() && Elam.empty()) break;
+    // end of stage: conv_test (pqdts.f90:2139) and the per-stage output event
+    if (!E.empty()) conv(E, X, G, false);
+    if (!Elam.empty()) conv(Elam, X, G, true);
+    fetch();
+    std::vector<int> both = E;
+    both.insert(both.end(), Elam.begin(), Elam.end());
+    for (int p : both) {
+      PState &q = *st[p];
+      q.conv = convval(p);
+      if (q.output >= 3 && q.cb) {
+        std::vector<double> lam(ctx[p]->M), lmin(ctx[p]->M);
+        HIPC(hipMemcpy(lam.data(), ctx[p]->lam, sizeof(double) * ctx[p]->M, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(lmin.data(), ctx[p]->lmin, sizeof(double) * ctx[p]->M, hipMemcpyDeviceToHost));
+        for (int i = 0; i < ctx[p]->M; i++) q.emit(PQDTS_EV_LAMBDA, {(double)(i + 1), lam[i], lmin[i]});
+      }
+      q.emit(PQDTS_EV_STAGE, {(double)q.stage, q.conv});
+      q.stage++;
+      if (q.stage > 2) {
+        q.phase = 2;
+      } else {
+        q.iiter = 0;
+        q.dlalpha = -std::numeric_limits<double>::max();
+        q.stage_exit = false;
+      }
+    }
+  }
+  std::vector<int> A, A1, A2;
+  for (int p = 0; p < cap; p++)
+    if (ctx[p] && st[p] && st[p]->phase == 1) {
+      A.push_back(p);
+      PState &q = *st[p];
+      q.iiter++;
+      q.newton++;
+      if (q.mode == 1) {
+        ctx[p]->full = false;
+        A1.push_back(p);
+      } else {
+        ctx[p]->full = (q.stage == 1);
+        A2.push_back(p);
+      }
+    }
+  if (A.empty()) return;
+  if (!A2.empty()) transform(A2, true);   // pqdts.f90:1970-1983
+  if (!A1.empty()) transform(A1, false);  // pqdts.f90:1940-1944
+  dL(A, X, G);
+  if (!A2.empty()) {
+    cg_init(A2);  // pqdts.f90:1986-2031
+    fetch();
+    for (int p : A2) {
+      PState &q = *st[p];
+      q.emit(PQDTS_EV_ACTIVE, {host_scal(p)[S_NACT], (double)ctx[p]->MN});
+      const double nd = host_scal(p)[S_ND];
+      q.eps_cg = std::fmin(0.5, std::sqrt(std::sqrt(nd))) * std::sqrt(nd);
+      q.ib1 = S_B1;
+    }
+    // PCG, pqdts.f90:2036-2090; every problem leaves on its own exit test
+    std::vector<int> C = A2;
+    for (int j = 1; j <= 50 && !C.empty(); j++) {
+      BF(C, XT, HP, true, true, true);
+      Scope sc(this, "CG");
+      const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
+      // within one round every problem shares the scalar slots
+      cg_update(C, st[C[0]]->ib1, i_rz);
+      cg_dir(C, st[C[0]]->ib1, i_rz);
+      fetch();
+      std::vector<int> next;
+      for (int p : C) {
+        PState &q = *st[p];
+        const double *h = host_scal(p);
+        const double b1 = h[q.ib1], v2 = h[i_rz], b2 = h[S_RR];
+        q.pcg++;
+        q.emit(PQDTS_EV_PCG, {(double)j, std::sqrt(b2), q.eps_cg, b1, v2});
+        q.ib1 = i_rz;
+        if (!(std::sqrt(b2) <= q.eps_cg)) next.push_back(p);
+      }
+      C.swap(next);
+    }
+    dL(A2, X, G);       // pqdts.f90:2093
+    twometric(A2);      // pqdts.f90:2095-2102
+  }
+  if (!A1.empty()) axpby(A1, SN, G, -1, -1.0, 0.0, AX_SCALE, false);  // pqdts.f90:1950
+  line_search(A);
+  // step, projection and re-evaluation, pqdts.f90:2111-2128
+  std::vector<int> stepS;
+  for (int p : A)
+    if (st[p]->alpha != 0.0) {
+      arg[p] = st[p]->alpha;
+      stepS.push_back(p);
+    }
+  if (!stepS.empty()) axpby(stepS, X, SN, X, 0.0, 0.0, AX_AXPY, true);
+  project(A, X, false, false);
+  L(A, X, true);
+  dL(A, X, G);
+  pdot(A, G, G);
+  conv(A, X, G, false);
+  fetch();
+  for (int p : A) {
+    PState &q = *st[p];
+    const double *h = host_scal(p);
+    q.conv = convval(p);
+    // xmin of doproj2 (0 where Condat projected: its rows write 0 partials)
+    q.emit(PQDTS_EV_ITER, {(double)q.stage, (double)q.iiter, Lval(p), std::fmin(0.0, h[S_XMIN]), q.conv,
+                           std::sqrt(h[S_DOT]), (double)q.mode, q.alpha, q.dlalpha});
+    if (q.conv < q.eps) q.stage_exit = true;
+  }
+}
+
+// ===========================================================================
+// context creation
+// ===========================================================================
+
 // Packs the rows of a banded matrix into per-tile dense transposed bands for
-// k_band_gemm: tile t covers rows [t*TILE, (t+1)*TILE); its band starts at
+// kb_band_gemm: tile t covers rows [t*TILE, (t+1)*TILE); its band starts at
 // column klo[t] and holds nch[t]*KC zero-filled columns; element (row r, col k)
 // of the tile sits at off[t] + (k - klo[t])*TILE + (r - t*TILE).
 struct HostBand {
@@ -680,6 +810,23 @@ struct HostBand {
   std::vector<int64_t> off;
   std::vector<int> klo, nch;
 };
+
+template <class Span>
+int64_t band_elems(int R, Span span) {
+  int64_t total = 0;
+  for (int t = 0; t < cdiv(R, TILE); t++) {
+    int lo = INT32_MAX, hi = -1;
+    for (int r = t * TILE; r < std::min(R, (t + 1) * TILE); r++) {
+      const auto sp = span(r);
+      if (sp.second > 0) {
+        lo = std::min(lo, sp.first);
+        hi = std::max(hi, sp.first + sp.second - 1);
+      }
+    }
+    if (hi >= 0) total += (int64_t)cdiv(hi - lo + 1, KC) * KC * TILE;
+  }
+  return total;
+}
 
 template <class Span, class Val>
 HostBand pack_band(int R, Span span, Val val) {
@@ -715,9 +862,10 @@ HostBand pack_band(int R, Span span, Val val) {
   return b;
 }
 
-void upload_band(pqdts_ctx *c, pqdts_ctx::Band &dst, const HostBand &h) {
+void upload_band(pqdts_ctx *c, pqdts_ctx::DBand &dst, const HostBand &h) {
   dst.ntiles = (int)h.off.size();
-  dst.pack = dalloc<double>((int64_t)h.pack.size());
+  dst.elems = (int64_t)h.pack.size();
+  dst.pack = dalloc<double>(dst.elems);
   dst.off = dalloc<int64_t>(dst.ntiles);
   dst.klo = dalloc<int>(dst.ntiles);
   dst.nch = dalloc<int>(dst.ntiles);
@@ -725,24 +873,27 @@ void upload_band(pqdts_ctx *c, pqdts_ctx::Band &dst, const HostBand &h) {
   HIPC(hipMemcpyAsync(dst.off, h.off.data(), sizeof(int64_t) * dst.ntiles, hipMemcpyHostToDevice, c->s));
   HIPC(hipMemcpyAsync(dst.klo, h.klo.data(), sizeof(int) * dst.ntiles, hipMemcpyHostToDevice, c->s));
   HIPC(hipMemcpyAsync(dst.nch, h.nch.data(), sizeof(int) * dst.ntiles, hipMemcpyHostToDevice, c->s));
+  HIPC(hipStreamSynchronize(c->s));  // host vectors die with the caller
 }
 
-// Packed elements of the band tiles of a banded matrix (the MFMA work per output column).
-template <class Span>
-int64_t band_elems(int R, Span span) {
-  int64_t total = 0;
-  for (int t = 0; t < cdiv(R, TILE); t++) {
-    int lo = INT32_MAX, hi = -1;
-    for (int r = t * TILE; r < std::min(R, (t + 1) * TILE); r++) {
-      const auto sp = span(r);
-      if (sp.second > 0) {
-        lo = std::min(lo, sp.first);
-        hi = std::max(hi, sp.first + sp.second - 1);
-      }
-    }
-    if (hi >= 0) total += (int64_t)cdiv(hi - lo + 1, KC) * KC * TILE;
+void free_band(pqdts_ctx::DBand &b) {
+  void *p[] = {b.pack, b.pack32, b.off, b.klo, b.nch};
+  for (void *q : p)
+    if (q) (void)hipFree(q);
+  b = pqdts_ctx::DBand{};
+}
+
+// FP32 copies of the band packs used by the mixed-precision Hessian products.
+void ensure_fp32(pqdts_ctx *c) {
+  for (pqdts_ctx::DBand *b : {&c->bf, &c->bt, &c->bg}) {
+    if (!b->pack || b->pack32) continue;
+    b->pack32 = dalloc<float>(b->elems);
+    k_round_f32<<<std::min(4096, cdiv(b->elems, 256)), 256, 0, c->s>>>(b->pack, b->pack32, b->elems);
+    HIPC(hipGetLastError());
   }
-  return total;
+  HIPC(hipStreamSynchronize(c->s));
+  refresh_desc(c);
+  if (c->eng) c->eng->attach(0, c);
 }
 
 // Forms G = F^T F once, on the device: F is expanded to a dense D x M matrix and
@@ -755,24 +906,35 @@ void ensure_gram(pqdts_ctx *c) {
   try {
     Fd = dalloc<double>((int64_t)D * M);
     Gd = dalloc<double>((int64_t)M * M);
-    k_dense_rows<<<cdiv(D, WAVES_PER_BLOCK), BLOCK, 0, c->s>>>(c->flo, c->flen, c->foff, c->fval, D, M, Fd);
-    BandArgs a = band_args(c->bt);
-    a.nrows = M;
-    a.nk = D;
-    a.N = M;
-    a.B = Fd;
-    a.C = Gd;
-    dim3 grid(cdiv(M, TILE), c->bt.ntiles);
-    k_band_gemm<TILE, TILE, KC, B_PLAIN, E_STORE, false, false, false><<<grid, BLOCK, 0, c->s>>>(a);
+    // a temporary descriptor whose operands are the dense F (B) and G (C)
+    Prob d = c->desc;
+    d.N = M;
+    d.op[0] = Fd;
+    d.op[1] = Gd;
+    Prob *dd = dalloc<Prob>(1);
+    HIPC(hipMemcpyAsync(dd, &d, sizeof(Prob), hipMemcpyHostToDevice, c->s));
+    WorkList wl;
+    wl.np = 1;
+    wl.pid[0] = 0;
+    wl.flags[0] = F_FULL;
+    wl.a[0] = 0.0;
+    wl.boff[0] = 0;
+    wl.boff[1] = cdiv(D, WAVES_PER_BLOCK);
+    kb_dense_rows<<<wl.boff[1], BLOCK, 0, c->s>>>(dd, wl, Fd);
     HIPC(hipGetLastError());
-    std::vector<double> G((size_t)M * M);
-    HIPC(hipMemcpyAsync(G.data(), Gd, sizeof(double) * G.size(), hipMemcpyDeviceToHost, c->s));
+    wl.boff[1] = cdiv(M, TILE) * c->bt.ntiles;
+    // G = F^T F: band F^T (rows k, columns d), B = dense F (D x M) with N := M
+    GemmOps ops{BAND_FT, 0, 1, 0, 0, 0};
+    kb_band_gemm<double, KC, B_PLAIN, E_STORE, false, false><<<wl.boff[1], BLOCK, 0, c->s>>>(dd, wl, ops);
+    HIPC(hipGetLastError());
+    std::vector<double> Gh((size_t)M * M);
+    HIPC(hipMemcpyAsync(Gh.data(), Gd, sizeof(double) * Gh.size(), hipMemcpyDeviceToHost, c->s));
     HIPC(hipStreamSynchronize(c->s));
+    (void)hipFree(dd);
     HostBand h = pack_band(
         M, [&](int k) { return std::make_pair(c->glo[k], c->gw[k]); },
-        [&](int k, int l) { return G[(size_t)k * M + l]; });
+        [&](int k, int l) { return Gh[(size_t)k * M + l]; });
     upload_band(c, c->bg, h);
-    HIPC(hipStreamSynchronize(c->s));
   } catch (...) {
     if (Fd) (void)hipFree(Fd);
     if (Gd) (void)hipFree(Gd);
@@ -780,21 +942,44 @@ void ensure_gram(pqdts_ctx *c) {
   }
   (void)hipFree(Fd);
   (void)hipFree(Gd);
+  if (c->bf.pack32) ensure_fp32(c);
+  refresh_desc(c);
+  if (c->eng) c->eng->attach(0, c);
+}
+
+void destroy_ctx(pqdts_ctx *c) {
+  if (!c) return;
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  c->eng.reset();
+  free_band(c->bf);
+  free_band(c->bt);
+  free_band(c->bg);
+  void *ptrs[] = {c->flo,       c->flen,     c->foff, c->fval, c->fhas, c->kd_lo, c->kd_hi, c->P,
+                  c->O,         c->diag_full, c->diag_red, c->maxr, c->ik,  c->ev,    c->gm,    c->lam,
+                  c->lmin,      c->part,     c->scal, c->ddesc};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (double *p : c->slot)
+    if (p) (void)hipFree(p);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
 }
 
 pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *Fr, const int32_t *Fc,
                       const double *Fv, int64_t nP, const int32_t *Pr, const int32_t *Pc, const double *Pv,
                       double gamma) {
   require(M >= 1 && N >= 1 && D >= 1, "need M >= 1, N >= 1, D >= 1");
-  require(M * N < (int64_t)INT32_MAX * 8 && M < INT32_MAX && N < INT32_MAX && D < INT32_MAX, "problem too large");
+  require(M * N < (int64_t)INT32_MAX * 8 && D * N < (int64_t)INT32_MAX * 8 && M < INT32_MAX && N < INT32_MAX &&
+              D < INT32_MAX,
+          "problem too large");
   require(nF >= 0 && nP >= 0, "negative nnz");
   require(nF == 0 || (Fr && Fc && Fv), "null F arrays");
   require(nP == 0 || (Pr && Pc && Pv), "null P arrays");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Fail(-2, "no GPU visible to pqdts_amd");
 
-  // F ingest: O(nnz) span build with zero fill and last-wins duplicates
-  // (same result as the O(D*nnz) scan of pqdts.f90:1328-1364)
+  // F ingest: O(nnz) span build with zero fill and last-wins duplicates (the same
+  // spans as the O(D*nnz) scan of pqdts.f90:1328-1364)
   std::vector<int> lo(D, INT32_MAX), hi(D, -1), len(D, 0);
   std::vector<uint8_t> has(D, 0);
   for (int64_t e = 0; e < nF; e++) {
@@ -812,16 +997,20 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
   }
   std::vector<double> val(off[D], 0.0);
   for (int64_t e = 0; e < nF; e++) val[off[Fr[e]] + (Fc[e] - lo[Fr[e]])] = Fv[e];
-  // per-k d-range (column spans of F)
-  std::vector<int> kdlo(M, INT32_MAX), kdhi(M, -1);
+  // column spans of F (rows of F^T) and the row bands of G = F^T F
+  std::vector<int> kdlo(M, INT32_MAX), kdhi(M, -1), glo(M, INT32_MAX), ghi(M, -1), gw(M, 0);
   for (int d = 0; d < D; d++)
     for (int k = lo[d]; k < lo[d] + len[d]; k++) {
       kdlo[k] = std::min(kdlo[k], d);
       kdhi[k] = std::max(kdhi[k], d);
+      glo[k] = std::min(glo[k], lo[d]);
+      ghi[k] = std::max(ghi[k], lo[d] + len[d] - 1);
     }
-  for (int k = 0; k < M; k++)
+  for (int k = 0; k < M; k++) {
     if (kdhi[k] < 0) kdlo[k] = 0;
-  // F and F^T packed for the band GEMM
+    if (ghi[k] < 0) glo[k] = 0;
+    gw[k] = ghi[k] >= 0 ? ghi[k] - glo[k] + 1 : 0;
+  }
   HostBand hbf = pack_band(
       (int)D, [&](int d) { return std::make_pair(lo[d], len[d]); },
       [&](int d, int k) { return val[off[d] + (k - lo[d])]; });
@@ -846,34 +1035,16 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     c->MN = M * N;
     c->DN = D * N;
     c->gamma = gamma;
-    c->sync_timers = getenv("PQDTS_SYNC_TIMERS") && atoi(getenv("PQDTS_SYNC_TIMERS")) != 0;
-    HIPC(hipGetDevice(&c->device));
-    HIPC(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     c->nspan = off[D];
-    c->tile_elems = (int64_t)hbf.pack.size();
-    c->nrt = (int)hbf.off.size();
-    c->nct = (int)hbt.off.size();
-    // G = F^T F is banded: row k reaches from the first to the last column of
-    // every F row that covers k. The Gram HVP is chosen when its band tiles are
-    // clearly cheaper than the two F products (and G fits in HBM many times over).
-    c->glo.assign(M, INT32_MAX);
-    c->gw.assign(M, 0);
-    {
-      std::vector<int> ghi(M, -1);
-      for (int d = 0; d < D; d++)
-        for (int k = lo[d]; k < lo[d] + len[d]; k++) {
-          c->glo[k] = std::min(c->glo[k], lo[d]);
-          ghi[k] = std::max(ghi[k], lo[d] + len[d] - 1);
-        }
-      for (int k = 0; k < M; k++) {
-        if (ghi[k] < 0) c->glo[k] = 0;
-        c->gw[k] = ghi[k] >= 0 ? ghi[k] - c->glo[k] + 1 : 0;
-      }
-    }
-    c->gram_elems = band_elems((int)M, [&](int k) { return std::make_pair(c->glo[k], c->gw[k]); });
+    c->glo = glo;
+    c->gw = gw;
+    for (int w : gw) c->gram_nnz += w;
+    c->gram_elems = band_elems((int)M, [&](int k) { return std::make_pair(glo[k], gw[k]); });
+    // the Gram HVP when its band tiles clearly cost less than the two F products
     const int64_t fpath = (int64_t)hbf.pack.size() + (int64_t)hbt.pack.size();
     c->gram_auto = (double)c->gram_elems < 0.8 * (double)fpath && M * M * 8 < ((int64_t)64 << 30);
     if (const char *e = getenv("PQDTS_HVP")) c->hvp_pref = (e[0] == 'G') ? 2 : (e[0] == 'F') ? 1 : 0;
+    HIPC(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     c->flo = dalloc<int>(D);
     c->flen = dalloc<int>(D);
     c->foff = dalloc<int64_t>(D);
@@ -893,10 +1064,11 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     c->lam = dalloc<double>(M);
     c->lmin = dalloc<double>(M);
     const int gx = cdiv(N, TILE);
-    c->pcap = std::max({gx * c->nrt, gx * c->nct, cdiv(M, WAVES_PER_BLOCK), 4096});
-    c->part = dalloc<double>((int64_t)c->pcap * R_NREG);
-    c->scal = dalloc<double>(S_NSCAL);
-    HIPC(hipHostMalloc((void **)&c->hscal, sizeof(double) * S_NSCAL, hipHostMallocDefault));
+    const int nbt = std::max((int)hbf.off.size(), (int)hbt.off.size());
+    c->pcap = std::max({gx * nbt, cdiv(M, WAVES_PER_BLOCK), cdiv(c->MN, EBLOCK), 1});
+    c->part = dalloc<double>((int64_t)c->pcap * NREG);
+    c->scal = dalloc<double>(NSCAL);
+    c->ddesc = dalloc<Prob>(1);
     auto up = [&](void *dst, const void *src, size_t bytes) {
       if (bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->s));
     };
@@ -905,25 +1077,60 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     up(c->foff, off.data(), sizeof(int64_t) * D);
     up(c->fval, val.data(), sizeof(double) * off[D]);
     up(c->fhas, has.data(), D);
-    upload_band(c, c->bf, hbf);
-    upload_band(c, c->bt, hbt);
     up(c->kd_lo, kdlo.data(), sizeof(int) * M);
     up(c->kd_hi, kdhi.data(), sizeof(int) * M);
     up(c->P, Ph.data(), sizeof(double) * c->DN);
     for (int s = 0; s < 6; s++) HIPC(hipMemsetAsync(c->slot[s], 0, sizeof(double) * c->MN, c->s));
     HIPC(hipMemsetAsync(c->O, 0, sizeof(double) * c->DN, c->s));
     HIPC(hipMemsetAsync(c->ik, 0, c->MN, c->s));
-    HIPC(hipMemsetAsync(c->scal, 0, sizeof(double) * S_NSCAL, c->s));
-    k_bdiag<<<c->row_blocks(), BLOCK, 0, c->s>>>(c->M, c->flo, c->flen, c->foff, c->fval, c->kd_lo, c->kd_hi,
-                                                 gamma, c->diag_full, c->diag_red);
+    HIPC(hipMemsetAsync(c->scal, 0, sizeof(double) * NSCAL, c->s));
+    std::vector<int> m1(M, -1);
+    up(c->maxr, m1.data(), sizeof(int) * M);
+    HIPC(hipStreamSynchronize(c->s));
+    upload_band(c, c->bf, hbf);
+    upload_band(c, c->bt, hbt);
+    refresh_desc(c);
+    // Bdiag once (stage-constant; the reference recomputes it at every CG step)
+    WorkList wl;
+    wl.np = 1;
+    wl.pid[0] = 0;
+    wl.boff[0] = 0;
+    wl.boff[1] = cdiv(M, WAVES_PER_BLOCK);
+    wl.flags[0] = F_FULL;
+    wl.a[0] = 0.0;
+    kb_bdiag<<<wl.boff[1], BLOCK, 0, c->s>>>(c->ddesc, wl);
     HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(c->s));  // host vectors die at return
-    fill_maxr(c, -1);
+    HIPC(hipStreamSynchronize(c->s));
+    if (c->use_gram()) ensure_gram(c);
   } catch (...) {
     destroy_ctx(c);
     throw;
   }
   return c;
+}
+
+pqdts_ctx *checked(pqdts_ctx *c) {
+  require(c != nullptr, "null context");
+  return c;
+}
+
+}  // namespace
+
+Engine &pqdts_ctx::engine() {
+  if (!eng) {
+    eng.reset(new Engine());
+    eng->private_scal = true;
+    eng->init(1, s, false);
+    eng->attach(0, this);
+  }
+  return *eng;
+}
+
+namespace {
+
+double *slot_ptr(pqdts_ctx *c, int slot) {
+  require(slot >= 0 && slot < 6, "invalid slot id " + std::to_string(slot));
+  return c->slot[slot];
 }
 
 void copy_slot(pqdts_ctx *c, int slot, double *host, const double *src, int layout, bool to_device) {
@@ -956,12 +1163,20 @@ void copy_slot(pqdts_ctx *c, int slot, double *host, const double *src, int layo
   }
 }
 
-pqdts_ctx *checked(pqdts_ctx *c) {
-  require(c != nullptr, "null context");
-  return c;
-}
+// standalone operator on one context
+const std::vector<int> ONE{0};
+
+Engine &E1(pqdts_ctx *c) { return checked(c)->engine(); }
 
 }  // namespace
+
+// ===========================================================================
+// batches (served / batched form)
+// ===========================================================================
+struct pqdts_batch {
+  Engine eng;
+  std::vector<int> finished;  // slots finished since the last query
+};
 
 // ===========================================================================
 // C ABI
@@ -1047,21 +1262,33 @@ int pqdts_get_fullspace(pqdts_ctx *ctx, int *full) {
 }
 
 int pqdts_L(pqdts_ctx *ctx, int x, double *val) {
-  return guarded([&] { *val = op_L(checked(ctx), slot_ptr(ctx, x)); });
+  return guarded([&] {
+    Engine &e = E1(ctx);
+    slot_ptr(ctx, x);
+    e.L(ONE, x, true);
+    e.fetch();
+    *val = e.Lval(0);
+  });
 }
 
 int pqdts_dL(pqdts_ctx *ctx, int x, int g) {
   return guarded([&] {
-    op_dL(checked(ctx), slot_ptr(ctx, x), slot_ptr(ctx, g));
-    HIPC(hipStreamSynchronize(ctx->s));
+    Engine &e = E1(ctx);
+    slot_ptr(ctx, x);
+    slot_ptr(ctx, g);
+    e.dL(ONE, x, g);
+    HIPC(hipStreamSynchronize(e.s));
   });
 }
 
 int pqdts_BF(pqdts_ctx *ctx, int dd, int bx) {
   return guarded([&] {
+    Engine &e = E1(ctx);
+    slot_ptr(ctx, dd);
+    slot_ptr(ctx, bx);
     require(dd != bx, "BF input and output must differ");
-    op_BF(checked(ctx), slot_ptr(ctx, dd), slot_ptr(ctx, bx), false, false, false);
-    HIPC(hipStreamSynchronize(ctx->s));
+    e.BF(ONE, dd, bx, false, false, false);
+    HIPC(hipStreamSynchronize(e.s));
   });
 }
 
@@ -1069,7 +1296,8 @@ int pqdts_Bdiag(pqdts_ctx *ctx, double *diag) {
   return guarded([&] {
     checked(ctx);
     if (diag) {
-      HIPC(hipMemcpyAsync(diag, ctx->diag(), sizeof(double) * ctx->M, hipMemcpyDeviceToHost, ctx->s));
+      HIPC(hipMemcpyAsync(diag, ctx->full ? ctx->diag_full : ctx->diag_red, sizeof(double) * ctx->M,
+                          hipMemcpyDeviceToHost, ctx->s));
       HIPC(hipStreamSynchronize(ctx->s));
     }
   });
@@ -1077,94 +1305,156 @@ int pqdts_Bdiag(pqdts_ctx *ctx, double *diag) {
 
 int pqdts_doproj(pqdts_ctx *ctx, int x) {
   return guarded([&] {
-    op_doproj(checked(ctx), slot_ptr(ctx, x));
-    HIPC(hipStreamSynchronize(ctx->s));
+    Engine &e = E1(ctx);
+    slot_ptr(ctx, x);
+    // doproj is the Condat projection whatever the space (pqdts.f90:1120-1140)
+    const bool f = ctx->full;
+    ctx->full = true;
+    try {
+      e.project(ONE, x, false, false);
+      HIPC(hipStreamSynchronize(e.s));
+    } catch (...) {
+      ctx->full = f;
+      throw;
+    }
+    ctx->full = f;
   });
 }
 
 int pqdts_doproj2(pqdts_ctx *ctx, int x, double *xmin) {
-  return guarded([&] { *xmin = op_doproj2(checked(ctx), slot_ptr(ctx, x)); });
+  return guarded([&] {
+    Engine &e = E1(ctx);
+    slot_ptr(ctx, x);
+    require(!ctx->full, "doproj2 needs the reduced space (maxr set)");
+    e.project(ONE, x, false, false);
+    e.fetch();
+    *xmin = std::fmin(0.0, e.host_scal(0)[S_XMIN]);
+  });
 }
 
 int pqdts_conv_test(pqdts_ctx *ctx, int x, int g, double *conv) {
-  return guarded([&] { *conv = op_conv(checked(ctx), slot_ptr(ctx, x), slot_ptr(ctx, g), false); });
+  return guarded([&] {
+    Engine &e = E1(ctx);
+    slot_ptr(ctx, x);
+    slot_ptr(ctx, g);
+    e.conv(ONE, x, g, false);
+    e.fetch();
+    *conv = e.convval(0);
+  });
 }
 
 int pqdts_pdot(pqdts_ctx *ctx, int x, int y, double *val) {
-  return guarded([&] { *val = op_pdot(checked(ctx), slot_ptr(ctx, x), slot_ptr(ctx, y)); });
+  return guarded([&] {
+    Engine &e = E1(ctx);
+    slot_ptr(ctx, x);
+    slot_ptr(ctx, y);
+    e.pdot(ONE, x, y);
+    e.fetch();
+    *val = e.host_scal(0)[S_DOT];
+  });
 }
 
-#define BLAS1_BODY(z, x, y, a, b, kind)                  \
-  return guarded([&] {                                   \
-    op_axpby(checked(ctx), z, x, y, a, b, kind);         \
-    HIPC(hipStreamSynchronize(ctx->s));                  \
-  })
-
-int pqdts_paxpy(pqdts_ctx *ctx, double alpha, int x, int y) {
-  BLAS1_BODY(slot_ptr(ctx, y), slot_ptr(ctx, x), slot_ptr(ctx, y), alpha, 0.0, 1);
+// BLAS-1 on slots; an operand of -1 is not used by that kind
+static int blas1(pqdts_ctx *ctx, int z, int x, int y, double a, double b, int kind) {
+  return guarded([&] {
+    Engine &e = E1(ctx);
+    slot_ptr(ctx, z);
+    if (x >= 0) slot_ptr(ctx, x);
+    if (y >= 0) slot_ptr(ctx, y);
+    e.axpby(ONE, z, x, y, a, b, kind, false);
+    HIPC(hipStreamSynchronize(e.s));
+  });
 }
+
+int pqdts_paxpy(pqdts_ctx *ctx, double alpha, int x, int y) { return blas1(ctx, y, x, y, alpha, 0.0, AX_AXPY); }
 int pqdts_pgaxpy(pqdts_ctx *ctx, double alpha, int x, double beta, int y) {
-  BLAS1_BODY(slot_ptr(ctx, y), slot_ptr(ctx, x), slot_ptr(ctx, y), alpha, beta, 0);
+  return blas1(ctx, y, x, y, alpha, beta, AX_GAXPY);
 }
 int pqdts_pgaxpy2(pqdts_ctx *ctx, double alpha, int x, double beta, int y, int z) {
-  BLAS1_BODY(slot_ptr(ctx, z), slot_ptr(ctx, x), slot_ptr(ctx, y), alpha, beta, 0);
+  return blas1(ctx, z, x, y, alpha, beta, AX_GAXPY);
 }
-int pqdts_pset(pqdts_ctx *ctx, double alpha, int x) {
-  BLAS1_BODY(slot_ptr(ctx, x), nullptr, nullptr, alpha, 0.0, 4);
-}
-int pqdts_pcopy(pqdts_ctx *ctx, int x, int y) {
-  BLAS1_BODY(slot_ptr(ctx, y), slot_ptr(ctx, x), nullptr, 1.0, 0.0, 3);
-}
-int pqdts_pscale(pqdts_ctx *ctx, double alpha, int x) {
-  BLAS1_BODY(slot_ptr(ctx, x), slot_ptr(ctx, x), nullptr, alpha, 0.0, 2);
-}
+int pqdts_pset(pqdts_ctx *ctx, double alpha, int x) { return blas1(ctx, x, -1, -1, alpha, 0.0, AX_SET); }
+int pqdts_pcopy(pqdts_ctx *ctx, int x, int y) { return blas1(ctx, y, x, -1, 1.0, 0.0, AX_COPY); }
+int pqdts_pscale(pqdts_ctx *ctx, double alpha, int x) { return blas1(ctx, x, x, -1, alpha, 0.0, AX_SCALE); }
 int pqdts_pcopy_scale(pqdts_ctx *ctx, double alpha, int x, int y) {
-  BLAS1_BODY(slot_ptr(ctx, y), slot_ptr(ctx, x), nullptr, alpha, 0.0, 2);
+  return blas1(ctx, y, x, -1, alpha, 0.0, AX_SCALE);
 }
 
 int pqdts_transform(pqdts_ctx *ctx) {
   return guarded([&] {
-    op_transform(checked(ctx), true);
-    HIPC(hipStreamSynchronize(ctx->s));
+    Engine &e = E1(ctx);
+    require(!ctx->full, "the reduced-space transform needs tfullspace = false");
+    e.transform(ONE, true);
+    HIPC(hipStreamSynchronize(e.s));
+  });
+}
+
+static int linesearch_one(pqdts_ctx *ctx, bool condat, double *alpha, double *dlalpha) {
+  return guarded([&] {
+    Engine &e = E1(ctx);
+    if (!condat) require(!ctx->full, "Lalpha_min2 needs the reduced space (maxr set)");
+    const bool f = ctx->full;
+    ctx->full = condat;
+    e.st[0].reset(new PState());
+    try {
+      e.line_search(ONE);
+    } catch (...) {
+      ctx->full = f;
+      throw;
+    }
+    ctx->full = f;
+    *alpha = e.st[0]->alpha;
+    *dlalpha = e.st[0]->dlalpha;
+    e.st[0].reset();
   });
 }
 
 int pqdts_lalpha_min2(pqdts_ctx *ctx, double *alpha, double *dlalpha) {
-  return guarded([&] { op_linesearch(checked(ctx), false, alpha, dlalpha); });
+  return linesearch_one(ctx, false, alpha, dlalpha);
 }
 
 int pqdts_lalpha_min_auglag(pqdts_ctx *ctx, double *alpha, double *dlalpha) {
-  return guarded([&] { op_linesearch(checked(ctx), true, alpha, dlalpha); });
+  return linesearch_one(ctx, true, alpha, dlalpha);
 }
 
 int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage, int output, pqdts_event_cb cb,
                 void *user, double *conv) {
   return guarded([&] {
-    checked(ctx);
+    Engine &e = E1(ctx);
     require(mode == 1 || mode == 2, "mode must be 1 or 2");
     require(start_stage == 1 || start_stage == 2, "start_stage must be 1 or 2");
-    ctx->cb = cb;
-    ctx->user = user;
+    PState *q = new PState();
+    e.st[0].reset(q);
+    q->mode = mode;
+    q->maxiter = maxiter;
+    q->eps = eps;
+    q->start_stage = start_stage;
+    q->output = output;
+    q->cb = cb;
+    q->user = user;
+    e.fp32[0] = 0;
+    Engine::Scope sc(&e, "min");
+    // the reference starts in full space (pqdts.f90:1225)
+    ctx->full = true;
     try {
-      double cv = solve(ctx, mode, maxiter, eps, start_stage, output);
-      if (conv) *conv = cv;
+      while (e.any_active()) e.tick();
     } catch (...) {
-      ctx->cb = nullptr;
-      ctx->user = nullptr;
+      e.st[0].reset();
       throw;
     }
-    ctx->cb = nullptr;
-    ctx->user = nullptr;
+    HIPC(hipStreamSynchronize(e.s));
+    if (conv) *conv = q->conv;
+    e.st[0].reset();
   });
 }
 
 int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *seconds) {
   int n = 0;
   int rc = guarded([&] {
-    checked(ctx);
-    for (const auto &name : ctx->timer_order) {
+    Engine &e = E1(ctx);
+    for (const auto &name : e.timer_order) {
       if (n < cap) {
-        const Timer &t = ctx->timers[name];
+        const Timer &t = e.timers[name];
         if (names) {
           std::strncpy(names[n], name.c_str(), 23);
           names[n][23] = 0;
@@ -1178,35 +1468,38 @@ int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double
   return rc == 0 ? n : -1;
 }
 
-int pqdts_kernel_timing(pqdts_ctx *ctx, int on) {
-  return guarded([&] {
-    checked(ctx);
-    HIPC(hipStreamSynchronize(ctx->s));
-    ctx->kfold();
-    ctx->ktiming = on != 0;
-    for (size_t i = 0; i < ctx->kcount.size(); i++) {
-      ctx->kcount[i] = 0;
-      ctx->kms[i] = 0.0;
+static int kernel_stats_of(Engine &e, int cap, char (*names)[24], int *counts, double *ms) {
+  HIPC(hipStreamSynchronize(e.s));
+  e.kfold();
+  const int n = (int)e.knames.size();
+  for (int i = 0; i < std::min(n, cap); i++) {
+    if (names) {
+      std::strncpy(names[i], e.knames[i].c_str(), 23);
+      names[i][23] = 0;
     }
-  });
+    if (counts) counts[i] = e.kcount[i];
+    if (ms) ms[i] = e.kms[i];
+  }
+  return n;
+}
+
+static void kernel_timing_of(Engine &e, int on) {
+  HIPC(hipStreamSynchronize(e.s));
+  e.kfold();
+  e.ktiming = on != 0;
+  for (size_t i = 0; i < e.kcount.size(); i++) {
+    e.kcount[i] = 0;
+    e.kms[i] = 0.0;
+  }
+}
+
+int pqdts_kernel_timing(pqdts_ctx *ctx, int on) {
+  return guarded([&] { kernel_timing_of(E1(ctx), on); });
 }
 
 int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms) {
   int n = 0;
-  int rc = guarded([&] {
-    checked(ctx);
-    HIPC(hipStreamSynchronize(ctx->s));
-    ctx->kfold();
-    n = (int)ctx->knames.size();
-    for (int i = 0; i < std::min(n, cap); i++) {
-      if (names) {
-        std::strncpy(names[i], ctx->knames[i].c_str(), 23);
-        names[i][23] = 0;
-      }
-      if (counts) counts[i] = ctx->kcount[i];
-      if (ms) ms[i] = ctx->kms[i];
-    }
-  });
+  int rc = guarded([&] { n = kernel_stats_of(E1(ctx), cap, names, counts, ms); });
   return rc == 0 ? n : -1;
 }
 
@@ -1218,16 +1511,16 @@ int pqdts_info(pqdts_ctx *ctx, int64_t *info) {
     info[1] = ctx->N;
     info[2] = ctx->D;
     info[3] = ctx->nspan;
-    info[4] = ctx->tile_elems;
-    info[5] = ctx->nrt;
-    info[6] = ctx->nct;
+    info[4] = ctx->bf.elems;
+    info[5] = ctx->bf.ntiles;
+    info[6] = ctx->bt.ntiles;
     info[7] = ctx->use_gram() ? 1 : 0;
     info[8] = ctx->gram_elems;
-    int64_t gnnz = 0;
-    for (int w : ctx->gw) gnnz += w;
-    info[9] = gnnz;
-    info[10] = 0;
-    info[11] = 0;
+    info[9] = ctx->gram_nnz;
+    info[10] = ctx->bt.elems;
+    // device bytes held by this problem
+    info[11] = 8 * (6 * ctx->MN + 2 * ctx->DN + ctx->nspan + ctx->bf.elems + ctx->bt.elems + ctx->bg.elems) +
+               4 * (ctx->bf.pack32 ? ctx->bf.elems + ctx->bt.elems + ctx->bg.elems : 0) + ctx->MN;
   });
 }
 
@@ -1245,16 +1538,158 @@ int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n) 
     require(rows >= 0 && n >= 1 && rows * n < (int64_t)INT32_MAX * 8, "bad shape");
     require(y != nullptr && x != nullptr && x != y, "x must be a distinct non-null buffer");
     if (rows == 0) return;
+    // a context-free descriptor with the rows as its X operand
     double *d = dalloc<double>(rows * n);
+    Prob *pd = nullptr;
     hipError_t e1 = hipMemcpy(d, y, sizeof(double) * rows * n, hipMemcpyHostToDevice);
+    double *part = nullptr;
     if (e1 == hipSuccess) {
-      k_simplex_rows<<<cdiv(rows, WAVES_PER_BLOCK), BLOCK>>>(d, nullptr, nullptr, 0.0, (int)rows, (int)n, 0);
-      e1 = hipGetLastError();
+      Prob p{};
+      p.M = (int)rows;
+      p.N = (int)n;
+      p.pcap = cdiv(rows, WAVES_PER_BLOCK);
+      p.op[0] = d;
+      part = dalloc<double>((int64_t)p.pcap * NREG);
+      p.part = part;
+      pd = dalloc<Prob>(1);
+      e1 = hipMemcpy(pd, &p, sizeof(Prob), hipMemcpyHostToDevice);
+      if (e1 == hipSuccess) {
+        WorkList wl;
+        wl.np = 1;
+        wl.pid[0] = 0;
+        wl.boff[0] = 0;
+        wl.boff[1] = p.pcap;
+        wl.flags[0] = F_FULL;
+        wl.a[0] = 0.0;
+        kb_project<<<wl.boff[1], BLOCK>>>(pd, wl, 0, 0, 0, R_C);
+        e1 = hipGetLastError();
+      }
     }
     if (e1 == hipSuccess) e1 = hipMemcpy(x, d, sizeof(double) * rows * n, hipMemcpyDeviceToHost);
     (void)hipFree(d);
+    if (pd) (void)hipFree(pd);
+    if (part) (void)hipFree(part);
     HIPC(e1);
   });
+}
+
+// ---- batches --------------------------------------------------------------
+int pqdts_batch_create(int capacity, pqdts_batch **out) {
+  return guarded([&] {
+    require(out != nullptr && capacity >= 1, "bad batch arguments");
+    *out = nullptr;
+    pqdts_batch *b = new pqdts_batch();
+    try {
+      b->eng.init(capacity, nullptr, true);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int pqdts_batch_destroy(pqdts_batch *b) {
+  return guarded([&] { delete b; });
+}
+
+int pqdts_batch_admit(pqdts_batch *b, pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage,
+                      int output, int precision, pqdts_event_cb cb, void *user, int *slot) {
+  return guarded([&] {
+    require(b != nullptr, "null batch");
+    checked(ctx);
+    require(mode == 1 || mode == 2, "mode must be 1 or 2");
+    require(start_stage == 1 || start_stage == 2, "start_stage must be 1 or 2");
+    require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
+    Engine &e = b->eng;
+    int p = -1;
+    for (int i = 0; i < e.cap; i++)
+      if (!e.ctx[i]) {
+        p = i;
+        break;
+      }
+    require(p >= 0, "batch is full");
+    HIPC(hipStreamSynchronize(ctx->s));
+    if (precision) ensure_fp32(ctx);
+    e.attach(p, ctx);
+    PState *q = new PState();
+    q->mode = mode;
+    q->maxiter = maxiter;
+    q->eps = eps;
+    q->start_stage = start_stage;
+    q->output = output;
+    q->cb = cb;
+    q->user = user;
+    e.st[p].reset(q);
+    e.fp32[p] = precision;
+    ctx->full = true;
+    if (slot) *slot = p;
+  });
+}
+
+int pqdts_batch_step(pqdts_batch *b, int *finished, int cap) {
+  int n = 0;
+  int rc = guarded([&] {
+    require(b != nullptr, "null batch");
+    Engine &e = b->eng;
+    if (e.any_active()) e.tick();
+    HIPC(hipStreamSynchronize(e.s));
+    for (int p = 0; p < e.cap; p++)
+      if (e.ctx[p] && e.st[p] && e.st[p]->phase == 2) {
+        if (finished && n < cap) finished[n] = p;
+        n++;
+      }
+  });
+  return rc == 0 ? n : -1;
+}
+
+int pqdts_batch_result(pqdts_batch *b, int slot, double *conv, int *stats) {
+  return guarded([&] {
+    require(b != nullptr && slot >= 0 && slot < b->eng.cap && b->eng.ctx[slot] && b->eng.st[slot],
+            "no problem in that slot");
+    PState &q = *b->eng.st[slot];
+    if (conv) *conv = q.conv;
+    if (stats) {
+      stats[0] = q.phase == 2 ? 1 : 0;
+      stats[1] = q.newton;
+      stats[2] = q.pcg;
+      stats[3] = q.trials;
+    }
+  });
+}
+
+int pqdts_batch_release(pqdts_batch *b, int slot) {
+  return guarded([&] {
+    require(b != nullptr && slot >= 0 && slot < b->eng.cap, "bad slot");
+    HIPC(hipStreamSynchronize(b->eng.s));
+    b->eng.detach(slot);
+  });
+}
+
+int pqdts_batch_active(pqdts_batch *b) {
+  int n = 0;
+  int rc = guarded([&] {
+    require(b != nullptr, "null batch");
+    for (int p = 0; p < b->eng.cap; p++)
+      if (b->eng.ctx[p]) n++;
+  });
+  return rc == 0 ? n : -1;
+}
+
+int pqdts_batch_kernel_timing(pqdts_batch *b, int on) {
+  return guarded([&] {
+    require(b != nullptr, "null batch");
+    kernel_timing_of(b->eng, on);
+  });
+}
+
+int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms) {
+  int n = 0;
+  int rc = guarded([&] {
+    require(b != nullptr, "null batch");
+    n = kernel_stats_of(b->eng, cap, names, counts, ms);
+  });
+  return rc == 0 ? n : -1;
 }
 
 }  // extern "C"

@@ -1,0 +1,107 @@
+"""Batched / served form: many uneven problems advanced together.
+
+* Each problem's FP64 result is bit-identical alone (pqdts_solve) and in a batch
+  (same per-problem arithmetic order; the batch only changes which blocks run
+  together), including problems admitted while others are mid-solve.
+* Mixed precision (FP32 Hessian products inside CG, FP64 elsewhere) lands within
+  ||dPi||_inf <= 1e-6 of the FP64 solution when both are converged tightly, and
+  the relative objective gap is reported.
+"""
+import numpy as np
+import pytest
+
+import pqdts_amd
+from pqdts_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def problems():
+    return [
+        workloads.config1(),
+        workloads.random_problem(40, 15, 80, 0.7, 3e-3, 1e6, seed=11, eps=1e-7, name="a"),
+        workloads.random_problem(77, 131, 154, 0.9, 1e-5, 1e7, seed=12, eps=1e-7, name="b"),
+        workloads.random_problem(130, 60, 260, 0.6, 1e-2, 3e6, seed=13, eps=1e-7, name="c"),
+    ]
+
+
+def start(ctx, prob):
+    ctx.put("X", np.full((prob.M, prob.N), 1.0 / prob.N))
+
+
+def solve_alone(prob, maxiter=300):
+    ctx = pqdts_amd.Context.from_problem(prob)
+    start(ctx, prob)
+    conv, ev = ctx.solve(2, maxiter, prob.eps, 1, 0)
+    return ctx.get("X"), conv, [e for e in ev]
+
+
+def test_batch_equals_alone_bitwise():
+    probs = problems()
+    alone = [solve_alone(p) for p in probs]
+    b = pqdts_amd.Batch(8)
+    ctxs = []
+    for p in probs:
+        c = pqdts_amd.Context.from_problem(p)
+        start(c, p)
+        ctxs.append(c)
+        b.admit(c, 2, 300, p.eps, 1, 0, 0)
+    res = b.run()
+    for i, (p, c) in enumerate(zip(probs, ctxs)):
+        Xa, conva, eva = alone[i]
+        Xb = c.get("X")
+        assert np.array_equal(Xa, Xb), f"problem {p.name}: max diff {np.max(np.abs(Xa - Xb)):.3e}"
+        assert res[i]["conv"] == conva
+        evb = b.events(i)
+        assert [k for k, _ in evb] == [k for k, _ in eva]
+        assert all(np.array_equal(x[1], y[1]) for x, y in zip(evb, eva))
+
+
+def test_continuous_admission_matches_alone():
+    probs = problems()
+    alone = [solve_alone(p) for p in probs]
+    b = pqdts_amd.Batch(3)
+    ctxs = {}
+    pending = list(range(len(probs)))
+    done = {}
+    steps = 0
+    while pending or b.active():
+        # admit while slots are free
+        while pending and b.active() < 3:
+            i = pending.pop(0)
+            c = pqdts_amd.Context.from_problem(probs[i])
+            start(c, probs[i])
+            s = b.admit(c, 2, 300, probs[i].eps, 1, 0, 0)
+            ctxs[s] = (i, c)
+        for s in b.step():
+            i, c = ctxs.pop(s)
+            done[i] = c.get("X")
+            b.release(s)
+        steps += 1
+        assert steps < 5000
+    for i in range(len(probs)):
+        assert np.array_equal(done[i], alone[i][0]), probs[i].name
+
+
+@pytest.mark.parametrize("path", ["F", "G"])
+def test_mixed_precision_within_1e6_of_fp64(path):
+    p = workloads.config1()
+    eps = 1e-11
+    out = {}
+    for prec in (0, 1):
+        c = pqdts_amd.Context.from_problem(p)
+        c.set_hvp_path(path)
+        start(c, p)
+        b = pqdts_amd.Batch(1)
+        b.admit(c, 2, 3000, eps, 1, 0, prec)
+        r = b.run()[0]
+        its = [v for k, v in b.events(0) if k == 0]
+        out[prec] = (c.get("X"), its[-1][2], r)
+    (X64, L64, r64), (X32, L32, r32) = out[0], out[1]
+    dpi = float(np.max(np.abs(X64 - X32)))
+    gap = abs(L32 - L64) / L64
+    print(f"path {path}: ||dPi||inf = {dpi:.3e}, rel objective gap = {gap:.3e}, "
+          f"newton {r64['newton']} vs {r32['newton']}, pcg {r64['pcg']} vs {r32['pcg']}")
+    assert r64["conv"] < eps and r32["conv"] < eps
+    assert dpi <= 1e-6
+    assert gap <= 1e-9
