@@ -1,26 +1,38 @@
 #!/usr/bin/env python3
-"""Benchmark: converged FP64 solves per second of the detector-tomography
-problem M=2000, N=1000, D=4000 (alpha^2 on [0,1800], eta=0.75, 1000-pixel
-multiplexed read-out, 1e7 shots, gamma=1e-3, eps=1e-7, mode 2), one GPU.
+"""Benchmark of the detector-tomography solver on one MI355X.
 
-A step is one converged solve from Pi = 1/N with F and P already resident in
-HBM. Prints ONE JSON line on stdout; progress goes to stderr.
+Metric: converged solves/s (+ p99 latency), batched mixed precision vs plain FP64,
+with ||dPi||_inf <= 1e-6 between the two. Prints ONE JSON line on stdout;
+progress goes to stderr.
 
-roofline: the dominant kernel (largest device time over the timed region,
-measured with HIP events on the context's stream) with its algorithmic FP64
-work per launch, 2 * (stored F elements) * N flops, divided by its average
-launch duration.
+Workloads (--workload):
+  mix     (default) 256 independent problems of uneven size: M log-uniform in
+          [50, 2000], N uniform in [M/4, M], D = 2M, eta in [0.5, 0.95], gamma
+          log-uniform in [1e-5, 1e-2], 1e6-1e8 shots, solved to eps (default 1e-10)
+          so that mixed and FP64 results can be compared at 1e-6.
+          A step = the whole mix solved by the batched engine (one launch per
+          operator over all unfinished problems), mixed precision. `value` is its
+          converged solves/s. Beside it: the same engine in FP64, and the plain
+          FP64 solver run one problem at a time (pqdts_solve) on every
+          --seq-stride-th problem of the mix, with ||dPi||_inf and objective gaps
+          of the batched-mixed results against those plain FP64 results.
+  cfg2    one problem M=2000, N=1000, D=4000 (alpha^2 on [0,1800], eta=0.75,
+          1000-pixel read-out, 1e7 shots, gamma=1e-3, eps=1e-7): plain FP64 solves/s.
+  served  Poisson arrivals (50 req/s) of mix-distributed requests plus 10% large
+          ones (M=8000, N=4000, D=16000), admitted by HBM footprint into the
+          batched engine as they arrive; p50/p99 latency from arrival to result.
 
+Inputs are resident in HBM before timing starts (except in `served`, where the
+upload is part of serving a request). roofline: the dominant kernel over the
+timed region, its algorithmic work (2 x stored elements of its banded operand x
+N per problem and launch) over its HIP-event time on the engine's stream.
 cpu_baseline: the reference program (oracle/_ref/pqdts_omp.x, built from the
-upstream sources) in its own benchmark mode (argument 14 = 1) on the same
-inputs, host cores via OpenMP; its per-operator times are combined with this
-GPU solve's operator counts into a per-solve CPU time. Without the reference
-build, the CPU oracle is timed on a bounded prefix of the same solve instead.
+upstream sources) solving a bounded sample of the same problems on the host
+cores, or, without that build, the CPU oracle on a bounded sample.
 """
 import argparse
 import json
 import os
-import re
 import subprocess
 import sys
 import tempfile
@@ -32,20 +44,97 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "converged solves/s + p99 latency, batched mixed vs plain FP64, |ΔΠ|∞≤1e-6"
-FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
-HBM_PEAK_GBS = 8000.0
+FP64_PEAK = 78.6    # TFLOP/s, MI355X FP64 (vector and matrix)
+FP32_PEAK = 157.3   # TFLOP/s, MI355X FP32 matrix
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline_reference(prob, op_counts, threads):
-    """Reference bench mode on the same inputs; returns a cpu_baseline dict or None."""
+def pct(x, q):
+    return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
+
+
+def roofline_of(kstats):
+    """Dominant kernel by device time; achieved = its algorithmic flops / its time."""
+    if not kstats:
+        return None
+    dom = max(kstats, key=lambda k: kstats[k][1])
+    launches, ms, flops = kstats[dom]
+    peak = FP32_PEAK if dom.endswith("32") else FP64_PEAK
+    achieved = flops / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": None, "kernel": dom, "avg_launch_us": 1e3 * ms / max(launches, 1), "launches": launches,
+            "flops_per_launch": flops / max(launches, 1),
+            "kernels": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / max(v[0], 1),
+                            "tflops": (v[2] / (v[1] / 1e3) / 1e12) if v[1] > 0 else 0.0}
+                        for k, v in kstats.items()}}
+
+
+# ------------------------------------------------------------------ CPU baseline
+def ref_solve_seconds(prob, eps, maxiter, threads, timeout):
+    """Solves one problem with the reference binary; returns (wall s, final X) or None."""
+    import oracle
+    from pqdts_amd import fortio, workloads
+    with tempfile.TemporaryDirectory(dir="/tmp") as w:
+        NF, NP = workloads.write_inputs(w, prob)
+        cmd = [oracle.REF_BIN, str(prob.M), str(prob.N), str(prob.D), str(NF), str(NP), "2", str(maxiter), "1",
+               str(prob.gamma), str(eps), "1", "0", "0", "0"]
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="true", OMP_PLACES="cores")
+        t0 = time.time()
+        try:
+            subprocess.run(cmd, cwd=w, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=timeout, check=True)
+        except (subprocess.TimeoutExpired, subprocess.CalledProcessError):
+            return None
+        dt = time.time() - t0
+        _, _, _, _, _, X = fortio.read_state(os.path.join(w, "rank_     0_final.dat"))
+    return dt, X
+
+
+def cpu_baseline_mix(problems, order, eps, maxiter, threads, budget_s, gpu_seconds):
+    import oracle
+    if not os.path.exists(oracle.REF_BIN):
+        return cpu_baseline_port(problems[order[0]])
+    done, spent, names, gpu_t = 0, 0.0, [], 0.0
+    for i in order:
+        if spent > budget_s:
+            break
+        r = ref_solve_seconds(problems[i], eps, maxiter, threads, timeout=max(5.0, 2 * budget_s - spent))
+        if r is None:
+            break
+        spent += r[0]
+        done += 1
+        names.append(f"M={problems[i].M},N={problems[i].N}")
+        gpu_t += gpu_seconds.get(i, float("nan"))
+    if done == 0:
+        return None
+    return {"value": done / spent, "unit": "solves/s", "cores": threads, "kind": "reference",
+            "sample": (f"reference pqdts_omp.x ({threads} OpenMP threads) solving {done} problems of the mix "
+                       f"({', '.join(names)}) to eps={eps:g}: {spent:.1f} s; the plain FP64 GPU solver takes "
+                       f"{gpu_t:.2f} s for the same problems"),
+            "gpu_plain_fp64_same_sample_solves_per_s": done / gpu_t if gpu_t > 0 else None}
+
+
+def cpu_baseline_port(prob):
+    import oracle
+    o = oracle.Oracle.from_problem(prob)
+    o.buf("X")[:] = 1.0 / prob.N
+    t0 = time.time()
+    _, ev = o.solve(2, 1, prob.eps, 1, 0)
+    dt = time.time() - t0
+    n_pcg = sum(1 for k, _ in ev if k == oracle.EV_PCG)
+    return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle, 1 Newton iteration per stage of M={prob.M} ({n_pcg} PCG) in {dt:.1f} s"}
+
+
+def cpu_baseline_cfg2(prob, op_counts, threads):
+    """Reference bench mode (arg 14 = 1) per-operator times x this solve's operator counts."""
     import oracle
     from pqdts_amd import workloads
     if not os.path.exists(oracle.REF_BIN):
-        return None
+        return cpu_baseline_port(prob)
     with tempfile.TemporaryDirectory(dir="/tmp") as w:
         NF, NP = workloads.write_inputs(w, prob)
         cmd = [oracle.REF_BIN, str(prob.M), str(prob.N), str(prob.D), str(NF), str(NP), "2", "0", "0",
@@ -56,169 +145,326 @@ def cpu_baseline_reference(prob, op_counts, threads):
         r = subprocess.run(cmd, cwd=w, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                            timeout=600)
         wall = time.time() - t0
-    if r.returncode != 0:
-        log("reference bench failed:", r.stdout[-500:])
-        return None
     per = {}
     for line in r.stdout.splitlines():
         t = line.split()
         if len(t) >= 3 and t[0] == "t":
             per.setdefault(t[1], []).append(float(t[2]))
     med = {k: float(np.median(v)) for k, v in per.items()}
-    # one reference CG iteration = BF + Bdiag (recomputed, pqdts.f90:2065) + 4 pdot + 3 axpy-type sweeps
     n_bf, n_l, n_dl = op_counts.get("BF", 0), op_counts.get("L", 0), op_counts.get("dL", 0)
     n_proj = op_counts.get("doproj", 0)
+    # one reference CG step = BF + Bdiag (recomputed, pqdts.f90:2065) + 4 pdot + 3 axpy sweeps
     t_solve = (n_bf * (med["BF"] + med["Bdiag"] + 4 * med["pdot"] + 2 * med["paxpy"] + med["pgaxpy"])
                + n_l * med["L"] + n_dl * med["dL"] + n_proj * med["doproj"])
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "reference",
-            "sample": (f"reference pqdts_omp.x benchmark mode (arg 14 = 1) on the same inputs, "
-                       f"{threads} OpenMP threads, {wall:.1f} s wall incl. ingest; median per-operator times "
-                       f"(BF {med['BF']:.4f} s, dL {med['dL']:.4f} s, L {med['L']:.4f} s, Bdiag {med['Bdiag']:.4f} s) "
-                       f"times this solve's counts (BF {n_bf}, L {n_l}, dL {n_dl}, doproj {n_proj}) = "
-                       f"{t_solve:.1f} s per solve"),
-            "per_op_s": med}
+            "sample": (f"reference pqdts_omp.x benchmark mode (arg 14 = 1) on the same inputs, {threads} OpenMP "
+                       f"threads, {wall:.1f} s wall incl. ingest; median per-operator times (BF {med['BF']:.4f} s, "
+                       f"dL {med['dL']:.4f} s, L {med['L']:.4f} s, Bdiag {med['Bdiag']:.4f} s) x this solve's "
+                       f"counts (BF {n_bf}, L {n_l}, dL {n_dl}, doproj {n_proj}) = {t_solve:.1f} s per solve")}
 
 
-def cpu_baseline_port(prob, budget_s=20.0):
-    """The CPU oracle (scalar C) on a bounded prefix of the same solve."""
-    import oracle
-    o = oracle.Oracle.from_problem(prob)
-    o.buf("X")[:] = 1.0 / prob.N
-    t0 = time.time()
-    _, ev = o.solve(2, 1, prob.eps, 1, 0)   # one Newton iteration per stage
-    dt = time.time() - t0
-    n_pcg = sum(1 for k, _ in ev if k == oracle.EV_PCG)
-    return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle, 1 Newton iteration per stage ({n_pcg} PCG iterations) in {dt:.1f} s",
-            "seconds_per_pcg": dt / max(n_pcg, 1)}
+# ------------------------------------------------------------------ workloads
+def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing, tag):
+    """All problems through one batched engine from Pi = 1/N; returns (seconds, latencies, X list, stats)."""
+    for p, c in zip(problems, ctxs):
+        c.pset(1.0 / p.N, "X")
+    b = pqdts_amd.Batch(capacity)
+    if timing:
+        b.kernel_timing(True)
+    pending = list(range(len(problems)))
+    slot_of = {}
+    lat, X, res = [None] * len(problems), [None] * len(problems), [None] * len(problems)
+    t0 = time.perf_counter()
+    steps = 0
+    while pending or slot_of:
+        while pending and len(slot_of) < capacity:
+            i = pending.pop(0)
+            slot_of[b.admit(ctxs[i], 2, maxiter, problems[i].eps, 1, 0, precision, record=False)] = i
+        for s in b.step():
+            i = slot_of.pop(s)
+            lat[i] = time.perf_counter() - t0
+            res[i] = b.result(s)
+            b.release(s)
+        steps += 1
+        if steps % 100 == 0:
+            log(f"  {tag}: {steps} steps, {len(pending) + len(slot_of)} unfinished, "
+                f"{time.perf_counter() - t0:.1f} s")
+    dt = time.perf_counter() - t0
+    kst = b.kernel_stats() if timing else {}
+    for i, c in enumerate(ctxs):
+        X[i] = c.get("X")
+    b.close()
+    return dt, lat, X, res, kst
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--maxiter", type=int, default=2000)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--kernels", type=int, default=0,
-                    help="instead of solving: run each hot operator this many times on the same problem "
-                         "(plus a plain 16 MB copy for counter calibration) and exit; for profiler passes")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems shard "
-              f"one per GPU with no communication, see DESIGN.md", file=sys.stderr)
-        sys.exit(2)
+def objective(ctx):
+    return ctx.L("X")
 
-    import pqdts_amd
-    from pqdts_amd import workloads
+
+def workload_mix(args, pqdts_amd, workloads):
+    t = time.time()
+    problems = workloads.batch_mix(args.count, eps=args.eps)
+    log(f"generated {len(problems)} problems in {time.time() - t:.1f} s; M median "
+        f"{int(np.median([p.M for p in problems]))}, max {max(p.M for p in problems)}")
     if pqdts_amd.device_count() < 1:
-        log("no GPU visible")
-        sys.exit(1)
+        raise SystemExit("no GPU visible")
+    t = time.time()
+    ctxs = [pqdts_amd.Context.from_problem(p) for p in problems]
+    log(f"uploaded in {time.time() - t:.1f} s")
+    cap = args.capacity or len(problems)
+    for w in range(args.warmup):
+        run_batched(pqdts_amd, problems[:8], ctxs[:8], 1, args.maxiter, cap, False, f"warmup {w}")
+    mixed_runs = []
+    for s in range(args.steps):
+        dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, s == 0,
+                                            f"batched mixed step {s}")
+        log(f"step {s}: batched mixed {dt:.2f} s for {len(problems)} solves")
+        mixed_runs.append((dt, lat, Xm, res, kst))
+    total = sum(r[0] for r in mixed_runs)
+    dt_m, lat_m, X_m, res_m, kst_m = mixed_runs[0]
+    Lm = []
+    for c, x in zip(ctxs, X_m):
+        c.put("X", x)
+        Lm.append(objective(c))
+    dt_b64, lat_b64, X_b64, res_b64, _ = run_batched(pqdts_amd, problems, ctxs, 0, args.maxiter, cap, False,
+                                                     "batched fp64")
+    log(f"batched fp64: {dt_b64:.2f} s")
+    # plain FP64 one problem at a time, every seq_stride-th problem
+    seq_idx = list(range(0, len(problems), args.seq_stride))
+    seq_t, dpi, gap, conv_ok = {}, [], [], []
+    for n, i in enumerate(seq_idx):
+        p, c = problems[i], ctxs[i]
+        c.pset(1.0 / p.N, "X")
+        t0 = time.perf_counter()
+        conv, _ = c.solve(2, args.maxiter, p.eps, 1, 0)
+        seq_t[i] = time.perf_counter() - t0
+        x64 = c.get("X")
+        L64 = objective(c)
+        dpi.append(float(np.max(np.abs(x64 - X_m[i]))))
+        gap.append(abs(Lm[i] - L64) / abs(L64))
+        conv_ok.append(conv < p.eps)
+        if n % 8 == 0:
+            log(f"  plain fp64 {n + 1}/{len(seq_idx)}: M={p.M} N={p.N} {seq_t[i]:.3f} s, dPi {dpi[-1]:.2e}")
+    seq_total = sum(seq_t.values())
+    plain_rate = len(seq_idx) / seq_total
+    value = len(problems) * args.steps / total
+    # the batched-mixed latency of the problems in the plain sample, for the same-sample comparison
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 Hessian products / f64 elsewhere", "data": "synthetic",
+        "config": {"workload": f"mix of {len(problems)} uneven problems (M log-uniform [50,2000], N U[M/4,M], "
+                               f"D=2M, eta U[0.5,0.95], gamma log-U[1e-5,1e-2], shots 1e6-1e8), eps={args.eps:g}, "
+                               f"mode 2, batched engine capacity {cap}, mixed precision",
+                   "count": len(problems), "eps": args.eps, "capacity": cap},
+        "latency_s": {"p50": pct(lat_m, 50), "p99": pct(lat_m, 99), "max": max(lat_m)},
+        "plain_fp64_one_at_a_time": {
+            "solves_per_s": plain_rate, "sample": f"every {args.seq_stride}th problem ({len(seq_idx)} problems)",
+            "seconds": seq_total,
+            "latency_s_if_served_fifo": {"p50": pct(np.cumsum([seq_t[i] for i in seq_idx]), 50),
+                                         "p99": pct(np.cumsum([seq_t[i] for i in seq_idx]), 99)}},
+        "speedup_vs_plain_fp64": value / plain_rate,
+        "batched_fp64": {"solves_per_s": len(problems) / dt_b64, "p50": pct(lat_b64, 50), "p99": pct(lat_b64, 99)},
+        "accuracy_mixed_vs_fp64": {"max_dpi_inf": max(dpi), "median_dpi_inf": pct(dpi, 50),
+                                   "max_rel_objective_gap": max(gap), "problems_compared": len(dpi),
+                                   "all_converged": bool(all(conv_ok) and all(r["conv"] < p.eps for r, p in
+                                                                                  zip(res_m, problems)))},
+        "solve_stats_mixed": {"newton": int(sum(r["newton"] for r in res_m)),
+                              "pcg": int(sum(r["pcg"] for r in res_m)),
+                              "max_newton": int(max(r["newton"] for r in res_m))},
+        "roofline": roofline_of(kst_m),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        rng = np.random.default_rng(7)
+        order = [i for i in rng.permutation(seq_idx) if problems[i].M <= 400]
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            out["cpu_baseline"] = cpu_baseline_mix(problems, order, args.eps, args.maxiter, threads,
+                                                   args.cpu_budget, seq_t)
+        except Exception as e:
+            log("cpu baseline failed:", repr(e))
+    return out
+
+
+def workload_cfg2(args, pqdts_amd, workloads):
     t = time.time()
     prob = workloads.config2()
     log(f"generated cfg2: M={prob.M} N={prob.N} D={prob.D} nnz={prob.nnz} in {time.time() - t:.1f} s")
+    if pqdts_amd.device_count() < 1:
+        raise SystemExit("no GPU visible")
     ctx = pqdts_amd.Context.from_problem(prob)
     info = ctx.info()
-    x0 = 1.0 / prob.N
-
-    if args.kernels:
-        rng = np.random.default_rng(0)
-        X = rng.random((prob.M, prob.N))
-        ctx.put("X", X / X.sum(1, keepdims=True))
-        ctx.put("XTMP", rng.normal(size=(prob.M, prob.N)))
-        for r in range(args.kernels):
-            for s in ("G", "HP", "SN"):  # 48 MB of writes push X out of the 32 MB of L2
-                ctx.pset(0.0, s)
-            ctx.pcopy("X", "Z")          # 16 MB read + 16 MB write: counter calibration
-            ctx.L("X")
-            ctx.dL("X", "G")
-            ctx.BF("XTMP", "HP")
-        log(f"ran {args.kernels} x (pcopy, L, dL, BF) on cfg2")
-        return
 
     def progress(_c, kind, v):
         if kind == 0 and int(v[1]) % 25 == 0:
             log(f"  stage {int(v[0])} iteration {int(v[1])}: L={v[2]:.10e} conv={v[4]:.3e}")
 
-    def one_solve(tag):
-        ctx.pset(x0, "X")
+    def one(tag):
+        ctx.pset(1.0 / prob.N, "X")
         t0 = time.perf_counter()
         conv, ev = ctx.solve(2, args.maxiter, prob.eps, 1, 0, on_event=progress)
         dt = time.perf_counter() - t0
         its = [v for k, v in ev if k == 0]
         npcg = sum(1 for k, _ in ev if k == 1)
         nls = sum(1 for k, _ in ev if k == 2)
-        log(f"{tag}: {dt:.3f} s, {len(its)} Newton iterations, {npcg} PCG, {nls} line-search trials, "
+        log(f"{tag}: {dt:.3f} s, {len(its)} Newton, {npcg} PCG, {nls} line-search trials, "
             f"L={its[-1][2]:.16e} conv={conv:.3e}")
         return dt, conv, its, npcg, nls
 
     for w in range(args.warmup):
-        one_solve(f"warmup {w}")
+        one(f"warmup {w}")
     ctx.kernel_timing(True)
-    base_timers = ctx.timers()
+    base = ctx.timers()
     lat = []
     t_all = time.perf_counter()
     for s in range(args.steps):
-        dt, conv, its, npcg, nls = one_solve(f"step {s}")
+        dt, conv, its, npcg, nls = one(f"step {s}")
         lat.append(dt)
     total = time.perf_counter() - t_all
-    kstats = ctx.kernel_stats()
+    kst = ctx.kernel_stats()
     timers = ctx.timers()
-    op_counts = {k: (timers[k][0] - base_timers.get(k, (0, 0))[0]) // args.steps for k in timers}
-
-    # roofline of the dominant kernel
-    dom = max(kstats, key=lambda k: kstats[k][1])
-    launches, ms = kstats[dom]
-    avg_s = ms / 1e3 / launches
-    # algorithmic FP64 work per launch: stored elements of the banded operand x N columns x 2
-    work = {"spmm_F": 2.0 * info["nspan"] * prob.N, "spmmT_F": 2.0 * info["nspan"] * prob.N,
-            "hvp_G": 2.0 * info["gram_nnz"] * prob.N}
-    flops = work[dom]
-    achieved = flops / avg_s / 1e12
-    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom,
-            "avg_launch_us": avg_s * 1e6, "launches_per_step": launches / args.steps,
-            "flops_per_launch": flops,
-            "kernels": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / max(v[0], 1)}
-                        for k, v in kstats.items()}}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        try:
-            cpu = cpu_baseline_reference(prob, op_counts, threads)
-            if cpu is None:
-                cpu = cpu_baseline_port(prob)
-        except Exception as e:  # the baseline is reported, never fatal
-            log("cpu baseline failed:", repr(e))
-
-    lat_sorted = np.sort(np.array(lat))
+    counts = {k: (timers[k][0] - base.get(k, (0, 0))[0]) // args.steps for k in timers}
     out = {
-        "metric": METRIC,
-        "value": args.steps / total,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * total / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
+        "metric": METRIC, "value": args.steps / total, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "cfg2 plain FP64 solve, one problem at a time", "M": prob.M, "N": prob.N,
                    "D": prob.D, "nnzF": prob.nnz, "gamma": prob.gamma, "eps": prob.eps, "mode": 2,
-                   "eta": 0.75, "shots": 1e7, "npix": 1000,
                    "hvp": "gram" if info["gram"] else "F products"},
-        "latency_s": {"p50": float(np.percentile(lat_sorted, 50)), "p99": float(np.percentile(lat_sorted, 99)),
-                      "max": float(lat_sorted[-1])},
+        "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
         "solve": {"newton_iterations": len(its), "pcg_iterations": npcg, "line_search_trials": nls,
-                  "final_L": its[-1][2], "final_conv": conv, "converged": bool(conv < prob.eps),
-                  "op_counts": op_counts},
-        "roofline": roof,
-        "cpu_baseline": cpu,
+                  "final_L": its[-1][2], "final_conv": conv, "converged": bool(conv < prob.eps), "op_counts": counts},
+        "roofline": roofline_of(kst),
+        "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline_cfg2(prob, counts, args.cpu_threads or min(16, os.cpu_count() or 1))
+        except Exception as e:
+            log("cpu baseline failed:", repr(e))
+    return out
+
+
+def workload_served(args, pqdts_amd, workloads):
+    rng = np.random.default_rng(11)
+    n = args.count
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    small = workloads.batch_mix(n, seed=5, eps=args.eps)
+    large_idx = set(int(i) for i in rng.choice(n, size=max(1, n // 10), replace=False))
+    specs = [(8000, 4000, 16000, 0.8, 1e-3, 1e8, 900 + i, args.eps, f"large{i}") for i in sorted(large_idx)]
+    larges = dict(zip(sorted(large_idx), workloads.build_many(specs)))
+    reqs = [larges.get(i, small[i]) for i in range(n)]
+    log(f"served: {n} requests over {arrivals[-1]:.1f} s, {len(larges)} large")
+    if pqdts_amd.device_count() < 1:
+        raise SystemExit("no GPU visible")
+    budget = args.hbm_gb * 2**30
+
+    def footprint(p):
+        # Pi-sized work arrays, O and P, F, F^T and G bands (upper estimate)
+        return 8 * (6 * p.M * p.N + 2 * p.D * p.N + 4 * p.nnz) + p.M * p.N
+
+    def serve(precision, cap):
+        b = pqdts_amd.Batch(cap)
+        t0 = time.perf_counter()
+        nxt, used, live, lat = 0, 0, {}, [None] * n
+        queue = []
+        while nxt < n or queue or live:
+            now = time.perf_counter() - t0
+            while nxt < n and arrivals[nxt] <= now:
+                queue.append(nxt)
+                nxt += 1
+            while queue and len(live) < cap and used + footprint(reqs[queue[0]]) <= budget:
+                i = queue.pop(0)
+                c = pqdts_amd.Context.from_problem(reqs[i])
+                c.pset(1.0 / reqs[i].N, "X")
+                used += footprint(reqs[i])
+                live[b.admit(c, 2, args.maxiter, reqs[i].eps, 1, 0, precision, record=False)] = (i, c)
+            if not live:
+                if nxt < n:
+                    time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+                continue
+            for s in b.step():
+                i, c = live.pop(s)
+                lat[i] = time.perf_counter() - t0 - arrivals[i]
+                b.release(s)
+                c.close()
+                used -= footprint(reqs[i])
+        dt = time.perf_counter() - t0
+        b.close()
+        return dt, lat
+
+    dt_m, lat_m = serve(1, args.capacity or 256)
+    log(f"served mixed: {dt_m:.1f} s, p50 {pct(lat_m, 50):.2f} s, p99 {pct(lat_m, 99):.2f} s")
+    dt_p, lat_p = serve(0, 1)
+    log(f"served plain fp64 one at a time: {dt_p:.1f} s, p50 {pct(lat_p, 50):.2f} s, p99 {pct(lat_p, 99):.2f} s")
+    return {
+        "metric": METRIC, "value": n / dt_m, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * dt_m, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 Hessian products / f64 elsewhere", "data": "synthetic",
+        "config": {"workload": f"served stream: {n} requests, Poisson arrivals at {args.rate:g}/s, 10% large "
+                               f"(M=8000,N=4000,D=16000), eps={args.eps:g}, admission by HBM footprint "
+                               f"({args.hbm_gb} GiB budget)"},
+        "latency_s": {"p50": pct(lat_m, 50), "p99": pct(lat_m, 99), "max": max(lat_m)},
+        "plain_fp64_one_at_a_time": {"solves_per_s": n / dt_p, "p50": pct(lat_p, 50), "p99": pct(lat_p, 99)},
+        "speedup_vs_plain_fp64": (n / dt_m) / (n / dt_p),
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def kernels_probe(args, pqdts_amd, workloads):
+    """Each hot operator a few times on cfg2 plus a 16 MB copy, for profiler passes."""
+    prob = workloads.config2()
+    ctx = pqdts_amd.Context.from_problem(prob)
+    rng = np.random.default_rng(0)
+    X = rng.random((prob.M, prob.N))
+    ctx.put("X", X / X.sum(1, keepdims=True))
+    ctx.put("XTMP", rng.normal(size=(prob.M, prob.N)))
+    for r in range(args.kernels):
+        for s in ("G", "HP", "SN"):  # 48 MB of writes push X out of the 32 MB of L2
+            ctx.pset(0.0, s)
+        ctx.pcopy("X", "Z")          # 16 MB read + 16 MB write: counter calibration
+        ctx.L("X")
+        ctx.dL("X", "G")
+        ctx.BF("XTMP", "HP")
+    log(f"ran {args.kernels} x (pcopy, L, dL, BF) on cfg2")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["mix", "cfg2", "served"], default="mix")
+    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--eps", type=float, default=1e-10)
+    ap.add_argument("--maxiter", type=int, default=5000)
+    ap.add_argument("--capacity", type=int, default=0)
+    ap.add_argument("--seq-stride", type=int, default=8)
+    ap.add_argument("--rate", type=float, default=50.0)
+    ap.add_argument("--hbm-gb", type=float, default=200.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--kernels", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems shard "
+              f"one per GPU with no communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import pqdts_amd
+    from pqdts_amd import workloads
+    if args.kernels:
+        kernels_probe(args, pqdts_amd, workloads)
+        return
+    if args.workload == "cfg2":
+        if args.maxiter == 5000:
+            args.maxiter = 2000
+        out = workload_cfg2(args, pqdts_amd, workloads)
+    elif args.workload == "served":
+        out = workload_served(args, pqdts_amd, workloads)
+    else:
+        out = workload_mix(args, pqdts_amd, workloads)
     print(json.dumps(out), flush=True)
 
 

@@ -145,10 +145,13 @@ int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double
 /* Per-kernel device timing with HIP events on the context's stream: `on`
  * enables it and clears the totals; pqdts_kernel_stats returns the number of
  * kernel names ("spmm_F" = O = F*V, "spmmT_F" = 2 F^T W + stencil) and fills
- * launch counts and total milliseconds. No counterpart in the reference (its
+ * launch counts and total milliseconds ("hvp_G" = 2 G p with the Gram matrix;
+ * names ending in 32 are the FP32 mixed-precision variants). No counterpart in the reference (its
  * timers are the wall-clock ones above, pqdts.f90:103-186). */
 int pqdts_kernel_timing(pqdts_ctx *ctx, int on);
-int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms);
+/* flops (may be NULL): algorithmic work of the timed launches, 2 x stored
+ * elements of the banded operand x N per problem and launch */
+int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms, double *flops);
 
 /* Sizes of a context: info[0..11] = M, N, D, stored F elements (sum of row span
  * lengths, the reference's Frows storage), packed elements of F's band tiles,
@@ -191,7 +194,8 @@ int pqdts_batch_result(pqdts_batch *b, int slot, double *conv, int *stats);
 int pqdts_batch_release(pqdts_batch *b, int slot);
 int pqdts_batch_active(pqdts_batch *b);
 int pqdts_batch_kernel_timing(pqdts_batch *b, int on);
-int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms);
+int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms,
+                             double *flops);
 
 /* simplexproj_condat_ (condat_simplexproj.c:60) over many rows at once: rows of
  * length n from y (host, row-major) projected onto the simplex into x (host).

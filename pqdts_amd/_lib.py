@@ -81,7 +81,7 @@ def load():
     L.pqdts_timers.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
     L.pqdts_simplexproj_rows.argtypes = [dp, dp, i64, i64]
     L.pqdts_kernel_timing.argtypes = [vp, i32]
-    L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
+    L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp]
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
     L.pqdts_set_hvp_path.argtypes = [vp, i32]
     L.pqdts_batch_create.argtypes = [i32, C.POINTER(vp)]
@@ -92,7 +92,7 @@ def load():
     L.pqdts_batch_release.argtypes = [vp, i32]
     L.pqdts_batch_active.argtypes = [vp]
     L.pqdts_batch_kernel_timing.argtypes = [vp, i32]
-    L.pqdts_batch_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
+    L.pqdts_batch_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp]
     _lib = L
     return L
 
@@ -295,16 +295,8 @@ class Context:
         _check(load().pqdts_kernel_timing(self.h, int(bool(on))))
 
     def kernel_stats(self) -> dict:
-        """{kernel: (launches, total_ms)} from HIP events on the context's stream."""
-        L = load()
-        n = L.pqdts_kernel_stats(self.h, 0, None, None, None)
-        if n <= 0:
-            return {}
-        names = (C.c_char * 24 * n)()
-        counts = (C.c_int * n)()
-        ms = (C.c_double * n)()
-        L.pqdts_kernel_stats(self.h, n, C.cast(names, C.c_void_p), counts, ms)
-        return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], ms[i]) for i in range(n)}
+        """{kernel: (launches, total_ms, algorithmic flops)} from HIP events on the context's stream."""
+        return _kstats(load().pqdts_kernel_stats, self.h)
 
     def timers(self):
         L = load()
@@ -319,14 +311,15 @@ class Context:
 
 
 def _kstats(fn, h):
-    n = fn(h, 0, None, None, None)
+    n = fn(h, 0, None, None, None, None)
     if n <= 0:
         return {}
     names = (C.c_char * 24 * n)()
     counts = (C.c_int * n)()
     ms = (C.c_double * n)()
-    fn(h, n, C.cast(names, C.c_void_p), counts, ms)
-    return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], ms[i]) for i in range(n)}
+    fl = (C.c_double * n)()
+    fn(h, n, C.cast(names, C.c_void_p), counts, ms, fl)
+    return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], ms[i], fl[i]) for i in range(n)}
 
 
 class Batch:

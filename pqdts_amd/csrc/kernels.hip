@@ -67,7 +67,10 @@ struct Prob {
   Band bf, bt, bg;
   double *part;          // NREG partial regions of pcap doubles
   double *scal;          // NSCAL device scalars
+  double *cglog;         // CG_LOG_ROUNDS x 5 values logged by the device-side CG loop
 };
+
+constexpr int CG_LOG_ROUNDS = 50;
 
 constexpr int NREG = 4;
 constexpr int NSCAL = 16;
@@ -77,6 +80,7 @@ constexpr unsigned F_FULL = 1u;  // full space (tfullspace); otherwise reduced
 
 struct WorkList {
   int np;
+  int guard;  // scalar slot that must be nonzero for a problem to run (-1: none)
   int pid[MAXB];
   int boff[MAXB + 1];
   unsigned char flags[MAXB];
@@ -101,6 +105,7 @@ __device__ __forceinline__ int wl_find(const WorkList &wl, int bid) {
 #define BATCH_PROLOGUE()                                   \
   const int wi_ = wl_find(wl, blockIdx.x);                 \
   const Prob &P = probs[wl.pid[wi_]];                      \
+  if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;    \
   const int lb = blockIdx.x - wl.boff[wi_];                \
   const bool full = (wl.flags[wi_] & F_FULL) != 0;         \
   const double arg = wl.a[wi_];                            \
@@ -164,6 +169,7 @@ __global__ __launch_bounds__(BLOCK) void kb_reduce(const Prob *__restrict__ prob
                                                    int dst, int is_min) {
   __shared__ double sh[WAVES_PER_BLOCK];
   const Prob &P = probs[wl.pid[blockIdx.x]];
+  if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;
   const int n = wl.boff[blockIdx.x + 1] - wl.boff[blockIdx.x];
   const double *part = P.part + (int64_t)region * P.pcap;
   double v;
@@ -859,6 +865,40 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_dir(const Prob *__restrict__ prob
     own = wave_sum(own);
     if (lane_id() == 0) P.ev[j] = -(psum - own);
   }
+}
+
+// Start of the device-side CG loop: eps = min(0.5, ||g||^(1/2)) ||g|| from the
+// squared gradient norm (pqdts.f90:2028) and the per-problem "still iterating"
+// flag. One thread per work-list entry.
+__global__ void kb_cg_start(const Prob *__restrict__ probs, WorkList wl, int i_nd, int i_eps, int i_act,
+                            int i_n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= wl.np) return;
+  double *sc = probs[wl.pid[i]].scal;
+  const double nd = sc[i_nd];
+  sc[i_eps] = fmin(0.5, sqrt(sqrt(nd))) * sqrt(nd);
+  sc[i_act] = 1.0;
+  sc[i_n] = 0.0;
+}
+
+// End of CG round j (pqdts.f90:2084-2089): log (j, sqrt(b2), eps, b1, v2) and
+// leave the loop once sqrt(b2) <= eps. Problems that already left are skipped.
+__global__ void kb_cg_log(const Prob *__restrict__ probs, WorkList wl, int j, int i_b1, int i_rz, int i_rr,
+                          int i_eps, int i_act, int i_n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= wl.np) return;
+  const Prob &P = probs[wl.pid[i]];
+  double *sc = P.scal;
+  if (sc[i_act] == 0.0) return;
+  const double b2 = sqrt(sc[i_rr]);
+  double *row = P.cglog + (int64_t)(j - 1) * 5;
+  row[0] = (double)j;
+  row[1] = b2;
+  row[2] = sc[i_eps];
+  row[3] = sc[i_b1];
+  row[4] = sc[i_rz];
+  sc[i_n] = (double)j;
+  if (b2 <= sc[i_eps]) sc[i_act] = 0.0;
 }
 
 // Two-metric diagonal step (pqdts.f90:2095-2102): sn = -G/diag on active entries

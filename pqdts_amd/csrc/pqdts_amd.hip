@@ -63,7 +63,10 @@ constexpr int TILE = 64;  // rows and columns of a band-GEMM block tile
 constexpr int KC = 16;    // band chunk staged per LDS buffer (4 MFMA k-steps)
 
 // device scalar slots per problem
-enum { S_LA = 0, S_LB, S_XMIN, S_NACT, S_ND, S_B1, S_V2, S_RZ0, S_RZ1, S_RR, S_CONV, S_DOT };
+enum {
+  S_LA = 0, S_LB, S_XMIN, S_NACT, S_ND, S_B1, S_V2, S_RZ0, S_RZ1, S_RR, S_CONV, S_DOT,
+  S_EPSCG, S_CGACT, S_CGN  // device-side CG loop: eps, still-iterating flag, rounds done
+};
 // partial-sum regions per problem
 enum { R_A = 0, R_B = 1, R_C = 2 };
 
@@ -194,6 +197,9 @@ struct Engine {
   double *hscal = nullptr;        // pinned host copy
   bool private_scal = false;      // standalone: scalars live in the context
   std::vector<double> arg;        // per-slot scalar argument of the next launch
+  int guard = -1;                 // launches skip problems whose scal[guard] is 0
+  double *dlog = nullptr;         // device-side CG logs [cap][CG_LOG_ROUNDS][5]
+  double *hlog = nullptr;         // pinned host copy
   std::vector<int> fp32;          // per-slot: mixed precision Hessian products
   // timers
   std::map<std::string, Timer> timers;
@@ -204,12 +210,13 @@ struct Engine {
   struct KPair {
     hipEvent_t a, b;
     int kid;
+    double work;
   };
   std::vector<KPair> kpending;
   std::vector<hipEvent_t> kpool;
   std::vector<std::string> knames;
   std::vector<int> kcount;
-  std::vector<double> kms;
+  std::vector<double> kms, kwork;
   std::vector<std::unique_ptr<PState>> st;  // solver state per slot
 
   ~Engine();
@@ -228,6 +235,8 @@ struct Engine {
     }
     dprobs = dalloc<Prob>(cap);
     HIPC(hipHostMalloc((void **)&hscal, sizeof(double) * NSCAL * cap, hipHostMallocDefault));
+    dlog = dalloc<double>((int64_t)cap * CG_LOG_ROUNDS * 5);
+    HIPC(hipHostMalloc((void **)&hlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipHostMallocDefault));
     sync_timers = getenv("PQDTS_SYNC_TIMERS") && atoi(getenv("PQDTS_SYNC_TIMERS")) != 0;
   }
 
@@ -236,6 +245,7 @@ struct Engine {
   void attach(int slot, pqdts_ctx *c) {
     ctx[slot] = c;
     Prob d = c->desc;
+    d.cglog = dlog + (int64_t)slot * CG_LOG_ROUNDS * 5;
     if (!private_scal) {
       if (!dscal) dscal = dalloc<double>((int64_t)NSCAL * cap);
       d.scal = dscal + (int64_t)NSCAL * slot;
@@ -269,6 +279,7 @@ struct Engine {
     knames.push_back(name);
     kcount.push_back(0);
     kms.push_back(0.0);
+    kwork.push_back(0.0);
     return (int)knames.size() - 1;
   }
   hipEvent_t kevent() {
@@ -287,11 +298,18 @@ struct Engine {
     HIPC(hipEventRecord(a, s));
     return a;
   }
-  void kend(hipEvent_t a, const std::string &name) {
+  void kend(hipEvent_t a, const std::string &name, double work = 0.0) {
     if (!a) return;
     hipEvent_t b = kevent();
     HIPC(hipEventRecord(b, s));
-    kpending.push_back({a, b, kernel_id(name)});
+    kpending.push_back({a, b, kernel_id(name), work});
+  }
+  // algorithmic FP64/FP32 flops of one band product over S: 2 x stored elements
+  // of the banded operand (F for F V and F^T W, G for G p) x N
+  double gemm_work(const std::vector<int> &S, int band) const {
+    double w = 0.0;
+    for (int p : S) w += 2.0 * (double)(band == BAND_G ? ctx[p]->gram_nnz : ctx[p]->nspan) * ctx[p]->N;
+    return w;
   }
   void kfold() {
     for (auto &p : kpending) {
@@ -299,6 +317,7 @@ struct Engine {
       HIPC(hipEventElapsedTime(&ms, p.a, p.b));
       kcount[p.kid]++;
       kms[p.kid] += ms;
+      kwork[p.kid] += p.work;
       kpool.push_back(p.a);
       kpool.push_back(p.b);
     }
@@ -322,6 +341,7 @@ struct Engine {
   void launch(const std::vector<int> &S, NB nb, L fn) {
     for (size_t g = 0; g < S.size(); g += MAXB) {
       WorkList wl;
+      wl.guard = guard;
       wl.np = (int)std::min<size_t>(MAXB, S.size() - g);
       int tot = 0;
       for (int i = 0; i < wl.np; i++) {
@@ -373,7 +393,7 @@ struct Engine {
     launch(S, [&](int p) { return nb_gemm(p, ops.band); }, [&](const WorkList &wl, int tot) {
       kb_band_gemm<T, KC, BM, EM, MASK, DOT><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
     });
-    kend(e0, kname);
+    kend(e0, kname, gemm_work(S, ops.band));
   }
 
   // ================= operators (reference subroutine in brackets) =================
@@ -394,7 +414,7 @@ struct Engine {
     launch_reduce(S, [&](int p) { return nb_gemm(p, BAND_F); }, [&](const WorkList &wl, int tot) {
       kb_band_gemm<double, KC, B_PLAIN, E_LOSS, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
     }, R_A, S_LA);
-    kend(e0, "spmm_F");
+    kend(e0, "spmm_F", gemm_work(S, BAND_F));
     launch_reduce(S, [&](int p) { return std::max(0, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); },
                   [&](const WorkList &wl, int tot) { kb_smooth<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, R_B); }, R_B,
                   S_LB);
@@ -471,7 +491,7 @@ struct Engine {
         kb_band_gemm<T, KC, BM, E_HVP, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
       });
     }
-    kend(e0, kname);
+    kend(e0, kname, gemm_work(S, ops.band));
   }
 
   // pdot (pqdts.f90:589-614) -> S_DOT
@@ -569,8 +589,7 @@ struct PState {
   int phase = 0;  // 0 initial evaluation pending, 1 iterating, 2 done
   int stage = 1, iiter = 0;
   bool stage_exit = false;
-  double dlalpha = 0.0, alpha = 0.0, conv = 0.0, eps_cg = 0.0;
-  int ib1 = S_B1;
+  double dlalpha = 0.0, alpha = 0.0, conv = 0.0;
   // line search
   double ls_L0 = 0.0, ls_df = 0.0, ls_alpha = 0.0, ls_lm = 0.0, ls_amin = 0.0;
   // counters
@@ -593,6 +612,8 @@ Engine::~Engine() {
   if (dprobs) (void)hipFree(dprobs);
   if (dscal) (void)hipFree(dscal);
   if (hscal) (void)hipHostFree(hscal);
+  if (dlog) (void)hipFree(dlog);
+  if (hlog) (void)hipHostFree(hlog);
   if (own_stream && s) (void)hipStreamDestroy(s);
 }
 
@@ -737,35 +758,46 @@ void Engine::tick() {
   dL(A, X, G);
   if (!A2.empty()) {
     cg_init(A2);  // pqdts.f90:1986-2031
+    launch(A2, [](int) { return 1; }, [&](const WorkList &wl, int) {
+      kb_cg_start<<<cdiv(wl.np, 64), 64, 0, s>>>(dprobs, wl, S_ND, S_EPSCG, S_CGACT, S_CGN);
+    });
+    // PCG, pqdts.f90:2036-2090. The exit test runs on the device: every CG launch
+    // skips problems whose flag dropped, and the host only checks every few rounds.
+    {
+      Scope sc(this, "CG");
+      guard = S_CGACT;
+      int ib1 = S_B1;
+      for (int j = 1; j <= 50; j++) {
+        BF(A2, XT, HP, true, true, true);
+        const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
+        cg_update(A2, ib1, i_rz);
+        cg_dir(A2, ib1, i_rz);
+        const int jj = j, b1 = ib1;
+        launch(A2, [](int) { return 1; }, [&](const WorkList &wl, int) {
+          kb_cg_log<<<cdiv(wl.np, 64), 64, 0, s>>>(dprobs, wl, jj, b1, i_rz, S_RR, S_EPSCG, S_CGACT, S_CGN);
+        });
+        ib1 = i_rz;
+        if (j % 8 == 0 || j == 50) {
+          fetch();
+          bool any = false;
+          for (int p : A2) any = any || host_scal(p)[S_CGACT] != 0.0;
+          if (!any) break;
+        }
+      }
+      guard = -1;
+    }
     fetch();
+    HIPC(hipMemcpyAsync(hlog, dlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
     for (int p : A2) {
       PState &q = *st[p];
-      q.emit(PQDTS_EV_ACTIVE, {host_scal(p)[S_NACT], (double)ctx[p]->MN});
-      const double nd = host_scal(p)[S_ND];
-      q.eps_cg = std::fmin(0.5, std::sqrt(std::sqrt(nd))) * std::sqrt(nd);
-      q.ib1 = S_B1;
-    }
-    // PCG, pqdts.f90:2036-2090; every problem leaves on its own exit test
-    std::vector<int> C = A2;
-    for (int j = 1; j <= 50 && !C.empty(); j++) {
-      BF(C, XT, HP, true, true, true);
-      Scope sc(this, "CG");
-      const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
-      // within one round every problem shares the scalar slots
-      cg_update(C, st[C[0]]->ib1, i_rz);
-      cg_dir(C, st[C[0]]->ib1, i_rz);
-      fetch();
-      std::vector<int> next;
-      for (int p : C) {
-        PState &q = *st[p];
-        const double *h = host_scal(p);
-        const double b1 = h[q.ib1], v2 = h[i_rz], b2 = h[S_RR];
-        q.pcg++;
-        q.emit(PQDTS_EV_PCG, {(double)j, std::sqrt(b2), q.eps_cg, b1, v2});
-        q.ib1 = i_rz;
-        if (!(std::sqrt(b2) <= q.eps_cg)) next.push_back(p);
-      }
-      C.swap(next);
+      const double *h = host_scal(p);
+      q.emit(PQDTS_EV_ACTIVE, {h[S_NACT], (double)ctx[p]->MN});
+      const int nr = (int)h[S_CGN];
+      const double *lg = hlog + (int64_t)p * CG_LOG_ROUNDS * 5;
+      for (int r = 0; r < nr; r++) q.emit(PQDTS_EV_PCG, {lg[5 * r], lg[5 * r + 1], lg[5 * r + 2], lg[5 * r + 3],
+                                                         lg[5 * r + 4]});
+      q.pcg += nr;
     }
     dL(A2, X, G);       // pqdts.f90:2093
     twometric(A2);      // pqdts.f90:2095-2102
@@ -914,6 +946,7 @@ void ensure_gram(pqdts_ctx *c) {
     Prob *dd = dalloc<Prob>(1);
     HIPC(hipMemcpyAsync(dd, &d, sizeof(Prob), hipMemcpyHostToDevice, c->s));
     WorkList wl;
+    wl.guard = -1;
     wl.np = 1;
     wl.pid[0] = 0;
     wl.flags[0] = F_FULL;
@@ -1092,6 +1125,7 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     refresh_desc(c);
     // Bdiag once (stage-constant; the reference recomputes it at every CG step)
     WorkList wl;
+    wl.guard = -1;
     wl.np = 1;
     wl.pid[0] = 0;
     wl.boff[0] = 0;
@@ -1468,7 +1502,7 @@ int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double
   return rc == 0 ? n : -1;
 }
 
-static int kernel_stats_of(Engine &e, int cap, char (*names)[24], int *counts, double *ms) {
+static int kernel_stats_of(Engine &e, int cap, char (*names)[24], int *counts, double *ms, double *flops) {
   HIPC(hipStreamSynchronize(e.s));
   e.kfold();
   const int n = (int)e.knames.size();
@@ -1479,6 +1513,7 @@ static int kernel_stats_of(Engine &e, int cap, char (*names)[24], int *counts, d
     }
     if (counts) counts[i] = e.kcount[i];
     if (ms) ms[i] = e.kms[i];
+    if (flops) flops[i] = e.kwork[i];
   }
   return n;
 }
@@ -1490,6 +1525,7 @@ static void kernel_timing_of(Engine &e, int on) {
   for (size_t i = 0; i < e.kcount.size(); i++) {
     e.kcount[i] = 0;
     e.kms[i] = 0.0;
+    e.kwork[i] = 0.0;
   }
 }
 
@@ -1497,9 +1533,9 @@ int pqdts_kernel_timing(pqdts_ctx *ctx, int on) {
   return guarded([&] { kernel_timing_of(E1(ctx), on); });
 }
 
-int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms) {
+int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms, double *flops) {
   int n = 0;
-  int rc = guarded([&] { n = kernel_stats_of(E1(ctx), cap, names, counts, ms); });
+  int rc = guarded([&] { n = kernel_stats_of(E1(ctx), cap, names, counts, ms, flops); });
   return rc == 0 ? n : -1;
 }
 
@@ -1555,6 +1591,7 @@ int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n) 
       e1 = hipMemcpy(pd, &p, sizeof(Prob), hipMemcpyHostToDevice);
       if (e1 == hipSuccess) {
         WorkList wl;
+        wl.guard = -1;
         wl.np = 1;
         wl.pid[0] = 0;
         wl.boff[0] = 0;
@@ -1683,11 +1720,12 @@ int pqdts_batch_kernel_timing(pqdts_batch *b, int on) {
   });
 }
 
-int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms) {
+int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms,
+                             double *flops) {
   int n = 0;
   int rc = guarded([&] {
     require(b != nullptr, "null batch");
-    n = kernel_stats_of(b->eng, cap, names, counts, ms);
+    n = kernel_stats_of(b->eng, cap, names, counts, ms, flops);
   });
   return rc == 0 ? n : -1;
 }

@@ -23,7 +23,6 @@ import os
 import numpy as np
 import scipy.sparse as sps
 from scipy.special import gammaln
-from scipy.stats import binom
 
 POISSON_CUT = 1e-14
 
@@ -75,16 +74,16 @@ def poisson_rows(alpha2: np.ndarray, M: int, cut: float = POISSON_CUT) -> sps.cs
 
 def loss_matrix(M: int, eta: float, cut: float = 0.0) -> sps.csr_matrix:
     """B[i, k] = Binom(k; i, eta): i photons, each transmitted with probability eta."""
-    rows, cols, vals = [], [], []
-    for i in range(M):
-        k = np.arange(i + 1)
-        p = binom.pmf(k, i, eta)
-        keep = np.nonzero(p > cut)[0]
-        rows.append(np.full(keep.size, i))
-        cols.append(k[keep])
-        vals.append(p[keep])
-    return sps.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                          shape=(M, M))
+    if eta >= 1.0:
+        return sps.identity(M, format="csr")
+    i = np.arange(M, dtype=np.float64)[:, None]
+    k = np.arange(M, dtype=np.float64)[None, :]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        logp = (gammaln(i + 1) - gammaln(k + 1) - gammaln(np.maximum(i - k, 0) + 1)
+                + k * math.log(eta) + (i - k) * math.log1p(-eta))
+    B = np.where(k <= i, np.exp(logp), 0.0)
+    B[B <= cut] = 0.0
+    return sps.csr_matrix(B)
 
 
 def saturate(K: int, N: int) -> sps.csr_matrix:
@@ -147,33 +146,67 @@ def config2(seed: int = 0) -> Problem:
     return Problem("cfg2", M, N, D, F, P, 1e-3, 1e-7, pi, {"eta": 0.75, "shots": 1e7, "npix": npix})
 
 
+def thinned_outcomes(alpha2: np.ndarray, eta: float, N: int) -> np.ndarray:
+    """Outcome distribution of a loss (efficiency eta) + saturation (at N-1) detector
+    probed by coherent states: binomial thinning of Poisson(a) photons is
+    Poisson(eta a), so q[d, n] = Poisson(n; eta a_d) for n < N-1 and the tail at N-1."""
+    lam = eta * np.asarray(alpha2, dtype=np.float64)[:, None]
+    n = np.arange(N - 1, dtype=np.float64)[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        logp = n * np.log(np.where(lam > 0, lam, 1.0)) - lam - gammaln(n + 1)
+    q = np.exp(logp)
+    q[:, :][lam[:, 0] == 0] = 0.0
+    q[lam[:, 0] == 0, 0] = 1.0
+    tail = np.clip(1.0 - q.sum(1), 0.0, None)
+    return np.concatenate([q, tail[:, None]], axis=1)
+
+
 def random_problem(M: int, N: int, D: int, eta: float, gamma: float, shots: float,
                    seed: int, eps: float = 1e-6, name: str = "rand") -> Problem:
-    """Loss + saturation detector with alpha^2 = linspace(0, 0.9 M, D)."""
+    """Loss + saturation detector with alpha^2 = linspace(0, 0.9 M, D). P is drawn
+    from the untruncated outcome distribution (thinned_outcomes), as a detector
+    would record it; F models photon numbers below M only, so for the brightest
+    probes F Pi_true misses the Poisson tail above M-1 that P contains."""
     alpha2 = np.linspace(0.0, 0.9 * M, D)
     F = poisson_rows(alpha2, M)
-    pi = (loss_matrix(M, eta, 1e-300) @ saturate(M, N)).toarray()
-    P = sample_P(F @ pi, shots, seed)
+    pi = (loss_matrix(M, eta, 1e-300) @ saturate(M, N)).toarray() if M <= 2500 else None
+    P = sample_P(thinned_outcomes(alpha2, eta, N), shots, seed)
     return Problem(name, M, N, D, F, P, gamma, eps, pi,
                    {"eta": eta, "shots": shots, "seed": seed})
 
 
 def batch_mix(count: int = 256, seed: int = 4, m_lo: int = 50, m_hi: int = 2000,
-              eps: float = 1e-6) -> list[Problem]:
+              eps: float = 1e-6, workers: int = 0) -> list[Problem]:
     """Uneven instances: M log-uniform in [m_lo, m_hi], N uniform in [M/4, M], D = 2M,
     eta uniform in [0.5, 0.95], gamma log-uniform in [1e-5, 1e-2], shots log-uniform
     in [1e6, 1e8]."""
     rng = np.random.default_rng(seed)
-    out = []
+    specs = []
     for b in range(count):
         M = int(round(math.exp(rng.uniform(math.log(m_lo), math.log(m_hi)))))
         N = int(rng.integers(max(2, M // 4), M + 1))
         eta = float(rng.uniform(0.5, 0.95))
         gamma = float(math.exp(rng.uniform(math.log(1e-5), math.log(1e-2))))
         shots = float(round(math.exp(rng.uniform(math.log(1e6), math.log(1e8)))))
-        out.append(random_problem(M, N, 2 * M, eta, gamma, shots, seed * 100003 + b, eps,
-                                  name=f"mix{b}"))
-    return out
+        specs.append((M, N, 2 * M, eta, gamma, shots, seed * 100003 + b, eps, f"mix{b}"))
+    return build_many(specs, workers)
+
+
+def _build(spec):
+    M, N, D, eta, gamma, shots, seed, eps, name = spec
+    return random_problem(M, N, D, eta, gamma, shots, seed, eps, name=name)
+
+
+def build_many(specs, workers: int = 0) -> list[Problem]:
+    """random_problem for every spec, in worker processes (each spec is seeded on its own).
+    Workers are spawned (not forked), so this is safe after the GPU runtime started."""
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(specs) < 4:
+        return [_build(s) for s in specs]
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_build, specs, chunksize=2))
 
 
 # ---------------------------------------------------------------------------
