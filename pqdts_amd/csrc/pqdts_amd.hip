@@ -211,8 +211,11 @@ struct Engine {
     hipEvent_t a, b;
     int kid;
     double work;
+    int round;                                // CG round of a guarded launch, else 0
+    std::vector<std::pair<int, double>> parts;  // guarded launch: (slot, work) per problem
   };
-  std::vector<KPair> kpending;
+  std::vector<KPair> kpending, kdeferred;
+  int kround = 0;  // current CG round while CG launches are guarded
   std::vector<hipEvent_t> kpool;
   std::vector<std::string> knames;
   std::vector<int> kcount;
@@ -298,18 +301,28 @@ struct Engine {
     HIPC(hipEventRecord(a, s));
     return a;
   }
-  void kend(hipEvent_t a, const std::string &name, double work = 0.0) {
+  // algorithmic FP64/FP32 flops of one band product of problem p: 2 x stored
+  // elements of the banded operand (F for F V and F^T W, G for G p) x N
+  double gemm_work(int p, int band) const {
+    return 2.0 * (double)(band == BAND_G ? ctx[p]->gram_nnz : ctx[p]->nspan) * ctx[p]->N;
+  }
+  // Closes a timed launch over S. Inside the guarded CG loop a problem whose CG
+  // already stopped does no work, so its share is settled by kresolve() once the
+  // round counts are known.
+  void kend(hipEvent_t a, const std::string &name, const std::vector<int> *S = nullptr, int band = -1) {
     if (!a) return;
     hipEvent_t b = kevent();
     HIPC(hipEventRecord(b, s));
-    kpending.push_back({a, b, kernel_id(name), work});
-  }
-  // algorithmic FP64/FP32 flops of one band product over S: 2 x stored elements
-  // of the banded operand (F for F V and F^T W, G for G p) x N
-  double gemm_work(const std::vector<int> &S, int band) const {
-    double w = 0.0;
-    for (int p : S) w += 2.0 * (double)(band == BAND_G ? ctx[p]->gram_nnz : ctx[p]->nspan) * ctx[p]->N;
-    return w;
+    KPair k{a, b, kernel_id(name), 0.0, 0, {}};
+    if (S) {
+      if (guard >= 0 && kround > 0) {
+        k.round = kround;
+        for (int p : *S) k.parts.push_back({p, gemm_work(p, band)});
+      } else {
+        for (int p : *S) k.work += gemm_work(p, band);
+      }
+    }
+    kpending.push_back(std::move(k));
   }
   void kfold() {
     for (auto &p : kpending) {
@@ -320,8 +333,16 @@ struct Engine {
       kwork[p.kid] += p.work;
       kpool.push_back(p.a);
       kpool.push_back(p.b);
+      if (p.round > 0) kdeferred.push_back(std::move(p));
     }
     kpending.clear();
+  }
+  // Work of guarded CG launches: problem p ran rounds 1..scal[S_CGN].
+  void kresolve() {
+    for (auto &d : kdeferred)
+      for (auto &pw : d.parts)
+        if (d.round <= (int)host_scal(pw.first)[S_CGN]) kwork[d.kid] += pw.second;
+    kdeferred.clear();
   }
 
   // ---- scalars
@@ -393,7 +414,7 @@ struct Engine {
     launch(S, [&](int p) { return nb_gemm(p, ops.band); }, [&](const WorkList &wl, int tot) {
       kb_band_gemm<T, KC, BM, EM, MASK, DOT><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
     });
-    kend(e0, kname, gemm_work(S, ops.band));
+    kend(e0, kname, &S, ops.band);
   }
 
   // ================= operators (reference subroutine in brackets) =================
@@ -414,7 +435,7 @@ struct Engine {
     launch_reduce(S, [&](int p) { return nb_gemm(p, BAND_F); }, [&](const WorkList &wl, int tot) {
       kb_band_gemm<double, KC, B_PLAIN, E_LOSS, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
     }, R_A, S_LA);
-    kend(e0, "spmm_F", gemm_work(S, BAND_F));
+    kend(e0, "spmm_F", &S, BAND_F);
     launch_reduce(S, [&](int p) { return std::max(0, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); },
                   [&](const WorkList &wl, int tot) { kb_smooth<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, R_B); }, R_B,
                   S_LB);
@@ -491,7 +512,7 @@ struct Engine {
         kb_band_gemm<T, KC, BM, E_HVP, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
       });
     }
-    kend(e0, kname, gemm_work(S, ops.band));
+    kend(e0, kname, &S, ops.band);
   }
 
   // pdot (pqdts.f90:589-614) -> S_DOT
@@ -768,6 +789,7 @@ void Engine::tick() {
       guard = S_CGACT;
       int ib1 = S_B1;
       for (int j = 1; j <= 50; j++) {
+        kround = j;
         BF(A2, XT, HP, true, true, true);
         const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
         cg_update(A2, ib1, i_rz);
@@ -785,8 +807,10 @@ void Engine::tick() {
         }
       }
       guard = -1;
+      kround = 0;
     }
     fetch();
+    kresolve();
     HIPC(hipMemcpyAsync(hlog, dlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
     for (int p : A2) {
