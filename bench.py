@@ -57,20 +57,33 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x, dtype=np.float64), q)) if len(x) else None
 
 
+HBM_PEAK = 8000.0    # GB/s, MI355X HBM3E
+
+
 def roofline_of(kstats):
-    """Dominant kernel by device time; achieved = its algorithmic flops / its time."""
+    """Dominant kernel by device time. Band products are priced against the MFMA
+    peak of their precision (achieved = algorithmic flops / time), streaming
+    kernels against HBM (achieved = algorithmic bytes / time)."""
     if not kstats:
         return None
     dom = max(kstats, key=lambda k: kstats[k][1])
-    launches, ms, flops = kstats[dom]
-    peak = FP32_PEAK if dom.endswith("32") else FP64_PEAK
-    achieved = flops / (ms / 1e3) / 1e12 if ms > 0 else 0.0
-    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": None, "kernel": dom, "avg_launch_us": 1e3 * ms / max(launches, 1), "launches": launches,
-            "flops_per_launch": flops / max(launches, 1),
-            "kernels": {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / max(v[0], 1),
-                            "tflops": (v[2] / (v[1] / 1e3) / 1e12) if v[1] > 0 else 0.0}
-                        for k, v in kstats.items()}}
+    launches, ms, flops, nbytes = kstats[dom]
+    sec = ms / 1e3
+    per = {k: {"launches": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / max(v[0], 1),
+               "tflops": (v[2] / (v[1] / 1e3) / 1e12) if v[1] > 0 else 0.0,
+               "gbps": (v[3] / (v[1] / 1e3) / 1e9) if v[1] > 0 else 0.0} for k, v in kstats.items()}
+    if flops > 0:
+        peak = FP32_PEAK if dom.endswith("32") else FP64_PEAK
+        achieved = flops / sec / 1e12 if sec > 0 else 0.0
+        out = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s"}
+    else:
+        achieved = nbytes / sec / 1e9 if sec > 0 else 0.0
+        out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s"}
+    out.update({"frac": out["achieved"] / out["peak"], "traffic": None, "kernel": dom,
+                "avg_launch_us": 1e3 * ms / max(launches, 1), "launches": launches,
+                "flops_per_launch": flops / max(launches, 1), "bytes_per_launch": nbytes / max(launches, 1),
+                "kernels": per})
+    return out
 
 
 # ------------------------------------------------------------------ CPU baseline
@@ -372,6 +385,69 @@ def workload_cfg2(args, pqdts_amd, workloads):
     return out
 
 
+def workload_cfg3(args, pqdts_amd, workloads):
+    """The single large instance: FP64 and mixed precision, each a solve capped at
+    --maxiter Newton iterations per stage from Pi = 1/N, every kernel timed."""
+    t = time.time()
+    prob = workloads.config3()
+    log(f"generated cfg3: M={prob.M} N={prob.N} D={prob.D} nnz={prob.nnz} in {time.time() - t:.1f} s")
+    if pqdts_amd.device_count() < 1:
+        raise SystemExit("no GPU visible")
+    t = time.time()
+    ctx = pqdts_amd.Context.from_problem(prob)
+    t_up = time.time() - t
+    info = ctx.info()
+    log(f"context in {t_up:.1f} s: {info}")
+    runs = {}
+    for prec in (0, 1):
+        ctx.pset(1.0 / prob.N, "X")
+        b = pqdts_amd.Batch(1)
+        b.kernel_timing(True)
+        b.admit(ctx, 2, args.maxiter, prob.eps, 1, 0, prec, record=True)
+        t0 = time.perf_counter()
+        while b.active():
+            fin = b.step()
+            its = [v for k, v in b.events(0) if k == 0]
+            if its:
+                log(f"  {'mixed' if prec else 'fp64'}: stage {int(its[-1][0])} iteration {int(its[-1][1])} "
+                    f"L={its[-1][2]:.10e} conv={its[-1][4]:.3e} {time.perf_counter() - t0:.1f} s")
+            if fin:
+                break
+        dt = time.perf_counter() - t0
+        res = b.result(0)
+        kst = b.kernel_stats()
+        its = [v for k, v in b.events(0) if k == 0]
+        b.close()
+        x = ctx.get("X")
+        runs[prec] = {"seconds": dt, "res": res, "kst": kst, "X": x, "L": objective(ctx), "iters": len(its)}
+        log(f"{'mixed' if prec else 'fp64'}: {dt:.2f} s, {res['newton']} Newton, {res['pcg']} PCG, "
+            f"L={runs[prec]['L']:.12e}")
+    r64, r32 = runs[0], runs[1]
+    dpi = float(np.max(np.abs(r64["X"] - r32["X"])))
+    gap = abs(r32["L"] - r64["L"]) / abs(r64["L"])
+    same_active = bool(np.array_equal(r64["X"] < 1e-8, r32["X"] < 1e-8))
+    rate = r32["res"]["newton"] / r32["seconds"]
+    return {
+        "metric": METRIC, "value": rate, "unit": "Newton iterations/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * r32["seconds"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 Hessian products / f64 elsewhere", "data": "synthetic",
+        "config": {"workload": f"single large instance M={prob.M}, N={prob.N}, D={prob.D} (alpha^2 on [0,18000], "
+                               f"eta=0.9, saturation at N-1, 1e8 shots, gamma=1e-2), mode 2 from Pi=1/N, capped at "
+                               f"{args.maxiter} Newton iterations per stage", "nnzF": prob.nnz,
+                   "hvp": "gram" if info["gram"] else "F products", "device_bytes": info.get("device_bytes")},
+        "fp64": {"seconds": r64["seconds"], "newton": r64["res"]["newton"], "pcg": r64["res"]["pcg"],
+                 "L": r64["L"], "s_per_newton": r64["seconds"] / max(r64["res"]["newton"], 1),
+                 "roofline": roofline_of(r64["kst"])},
+        "mixed": {"seconds": r32["seconds"], "newton": r32["res"]["newton"], "pcg": r32["res"]["pcg"],
+                  "L": r32["L"], "s_per_newton": r32["seconds"] / max(r32["res"]["newton"], 1)},
+        "accuracy_mixed_vs_fp64": {"dpi_inf": dpi, "rel_objective_gap": gap, "same_active_set_1e-8": same_active,
+                                   "note": "both capped at the same Newton iteration counts per stage"},
+        "upload_s": t_up,
+        "roofline": roofline_of(r32["kst"]),
+        "cpu_baseline": None,
+    }
+
+
 def workload_served(args, pqdts_amd, workloads):
     rng = np.random.default_rng(11)
     n = args.count
@@ -461,7 +537,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["mix", "cfg2", "served"], default="mix")
+    ap.add_argument("--workload", choices=["mix", "cfg2", "cfg3", "served"], default="mix")
     ap.add_argument("--count", type=int, default=256)
     ap.add_argument("--eps", type=float, default=1e-7)
     ap.add_argument("--maxiter", type=int, default=5000)
@@ -487,6 +563,10 @@ def main():
         if args.maxiter == 5000:
             args.maxiter = 2000
         out = workload_cfg2(args, pqdts_amd, workloads)
+    elif args.workload == "cfg3":
+        if args.maxiter == 5000:
+            args.maxiter = 3
+        out = workload_cfg3(args, pqdts_amd, workloads)
     elif args.workload == "served":
         out = workload_served(args, pqdts_amd, workloads)
     else:

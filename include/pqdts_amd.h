@@ -144,14 +144,19 @@ int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double
 
 /* Per-kernel device timing with HIP events on the context's stream: `on`
  * enables it and clears the totals; pqdts_kernel_stats returns the number of
- * kernel names ("spmm_F" = O = F*V, "spmmT_F" = 2 F^T W + stencil) and fills
- * launch counts and total milliseconds ("hvp_G" = 2 G p with the Gram matrix;
- * names ending in 32 are the FP32 mixed-precision variants). No counterpart in the reference (its
- * timers are the wall-clock ones above, pqdts.f90:103-186). */
+ * kernel names and fills launch counts and total milliseconds. Band products:
+ * "spmm_F" = O = F*V, "spmmT_F" = 2 F^T W + stencil, "hvp_G" = 2 G p with the
+ * Gram matrix (names ending in 32: the FP32 mixed-precision variants).
+ * Streaming kernels: "cg_init", "cg_update", "cg_dir", "foldval_gram",
+ * "project", "conv_rows", "axpby". No counterpart in the reference (its timers
+ * are the wall-clock ones above, pqdts.f90:103-186). */
 int pqdts_kernel_timing(pqdts_ctx *ctx, int on);
-/* flops (may be NULL): algorithmic work of the timed launches, 2 x stored
- * elements of the banded operand x N per problem and launch */
-int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms, double *flops);
+/* flops, bytes (each may be NULL): algorithmic work of the timed launches.
+ * flops = 2 x stored elements of the banded operand x N per problem and band
+ * product; bytes = each operand read once and each result written once (the
+ * banded operand at its stored precision, M x N and D x N arrays in FP64). */
+int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms, double *flops,
+                       double *bytes);
 
 /* Sizes of a context: info[0..11] = M, N, D, stored F elements (sum of row span
  * lengths, the reference's Frows storage), packed elements of F's band tiles,
@@ -195,7 +200,7 @@ int pqdts_batch_release(pqdts_batch *b, int slot);
 int pqdts_batch_active(pqdts_batch *b);
 int pqdts_batch_kernel_timing(pqdts_batch *b, int on);
 int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms,
-                             double *flops);
+                             double *flops, double *bytes);
 
 /* simplexproj_condat_ (condat_simplexproj.c:60) over many rows at once: rows of
  * length n from y (host, row-major) projected onto the simplex into x (host).

@@ -81,7 +81,7 @@ def load():
     L.pqdts_timers.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
     L.pqdts_simplexproj_rows.argtypes = [dp, dp, i64, i64]
     L.pqdts_kernel_timing.argtypes = [vp, i32]
-    L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp]
+    L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp, dp]
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
     L.pqdts_set_hvp_path.argtypes = [vp, i32]
     L.pqdts_batch_create.argtypes = [i32, C.POINTER(vp)]
@@ -92,7 +92,7 @@ def load():
     L.pqdts_batch_release.argtypes = [vp, i32]
     L.pqdts_batch_active.argtypes = [vp]
     L.pqdts_batch_kernel_timing.argtypes = [vp, i32]
-    L.pqdts_batch_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp]
+    L.pqdts_batch_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp, dp]
     _lib = L
     return L
 
@@ -285,7 +285,8 @@ class Context:
         a = (C.c_int64 * 12)()
         _check(load().pqdts_info(self.h, a))
         return {"M": a[0], "N": a[1], "D": a[2], "nspan": a[3], "tile_elems": a[4], "row_tiles": a[5],
-                "col_tiles": a[6], "gram": bool(a[7]), "gram_tile_elems": a[8], "gram_nnz": a[9]}
+                "col_tiles": a[6], "gram": bool(a[7]), "gram_tile_elems": a[8], "gram_nnz": a[9],
+                "ft_tile_elems": a[10], "device_bytes": a[11]}
 
     def set_hvp_path(self, path):
         """0 automatic, 1 the two F products, 2 the Gram matrix G = F^T F."""
@@ -295,7 +296,8 @@ class Context:
         _check(load().pqdts_kernel_timing(self.h, int(bool(on))))
 
     def kernel_stats(self) -> dict:
-        """{kernel: (launches, total_ms, algorithmic flops)} from HIP events on the context's stream."""
+        """{kernel: (launches, total_ms, algorithmic flops, algorithmic bytes)} from HIP events on the
+        context's stream."""
         return _kstats(load().pqdts_kernel_stats, self.h)
 
     def timers(self):
@@ -311,15 +313,16 @@ class Context:
 
 
 def _kstats(fn, h):
-    n = fn(h, 0, None, None, None, None)
+    n = fn(h, 0, None, None, None, None, None)
     if n <= 0:
         return {}
     names = (C.c_char * 24 * n)()
     counts = (C.c_int * n)()
     ms = (C.c_double * n)()
     fl = (C.c_double * n)()
-    fn(h, n, C.cast(names, C.c_void_p), counts, ms, fl)
-    return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], ms[i], fl[i]) for i in range(n)}
+    by = (C.c_double * n)()
+    fn(h, n, C.cast(names, C.c_void_p), counts, ms, fl, by)
+    return {bytes(names[i]).split(b"\0")[0].decode(): (counts[i], ms[i], fl[i], by[i]) for i in range(n)}
 
 
 class Batch:

@@ -68,6 +68,7 @@ struct Prob {
   double *part;          // NREG partial regions of pcap doubles
   double *scal;          // NSCAL device scalars
   double *cglog;         // CG_LOG_ROUNDS x 5 values logged by the device-side CG loop
+  int *pn;               // NREG: number of partials the last producer of each region stored
 };
 
 constexpr int CG_LOG_ROUNDS = 50;
@@ -107,9 +108,11 @@ __device__ __forceinline__ int wl_find(const WorkList &wl, int bid) {
   const Prob &P = probs[wl.pid[wi_]];                      \
   if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;    \
   const int lb = blockIdx.x - wl.boff[wi_];                \
+  const int nblk = wl.boff[wi_ + 1] - wl.boff[wi_];        \
   const bool full = (wl.flags[wi_] & F_FULL) != 0;         \
   const double arg = wl.a[wi_];                            \
   (void)full;                                              \
+  (void)nblk;                                              \
   (void)arg
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -161,17 +164,15 @@ __device__ __forceinline__ double block_min(double v, double *sh) {
 }
 
 // ---------------------------------------------------------------------------
-// Final reduction: launched with one block per work-list entry of the producing
-// launch; entry i had boff[i+1]-boff[i] blocks, whose partials it sums (or mins)
-// in index order into scal[dst].
+// Per-problem reductions. A producing launch stores one partial per block of a
+// problem (entry i of the work list has boff[i+1]-boff[i] blocks); the partials
+// are summed (or min-ed) in index order, the same order wherever that happens:
+//  * kb_reduce, one block per work-list entry, right after the producer;
+//  * kb_reduce_pn, several regions at once, counts taken from pn[region];
+//  * inside the next kernel, each of whose blocks sums the partials itself
+//    (reduce_parts_all): no extra launch and no inter-block synchronisation.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void kb_reduce(const Prob *__restrict__ probs, WorkList wl, int region,
-                                                   int dst, int is_min) {
-  __shared__ double sh[WAVES_PER_BLOCK];
-  const Prob &P = probs[wl.pid[blockIdx.x]];
-  if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;
-  const int n = wl.boff[blockIdx.x + 1] - wl.boff[blockIdx.x];
-  const double *part = P.part + (int64_t)region * P.pcap;
+__device__ __forceinline__ double reduce_parts(const double *part, int n, bool is_min, double *sh) {
   double v;
   if (is_min) {
     v = INFINITY;
@@ -182,7 +183,41 @@ __global__ __launch_bounds__(BLOCK) void kb_reduce(const Prob *__restrict__ prob
     for (int i = threadIdx.x; i < n; i += BLOCK) v += part[i];
     v = block_sum(v, sh);
   }
+  return v;
+}
+
+// reduce_parts with the result in every thread (`bc` is one shared double)
+__device__ __forceinline__ double reduce_parts_all(const double *part, int n, double *sh, double *bc) {
+  const double v = reduce_parts(part, n, false, sh);
+  if (threadIdx.x == 0) *bc = v;
+  __syncthreads();
+  const double r = *bc;
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_reduce(const Prob *__restrict__ probs, WorkList wl, int region,
+                                                   int dst, int is_min) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  const Prob &P = probs[wl.pid[blockIdx.x]];
+  if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;
+  const int n = wl.boff[blockIdx.x + 1] - wl.boff[blockIdx.x];
+  const double v = reduce_parts(P.part + (int64_t)region * P.pcap, n, is_min != 0, sh);
   if (threadIdx.x == 0) P.scal[dst] = v;
+}
+
+// Sums of up to three regions (scal[d_i] = sum of pn[r_i] partials of region r_i),
+// one block per work-list entry.
+__global__ __launch_bounds__(BLOCK) void kb_reduce_pn(const Prob *__restrict__ probs, WorkList wl, int nr, int r0,
+                                                      int d0, int r1, int d1, int r2, int d2) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  const Prob &P = probs[wl.pid[blockIdx.x]];
+  if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;
+  const int rr[3] = {r0, r1, r2}, dd[3] = {d0, d1, d2};
+  for (int k = 0; k < nr; k++) {
+    const double v = reduce_parts(P.part + (int64_t)rr[k] * P.pcap, P.pn[rr[k]], false, sh);
+    if (threadIdx.x == 0) P.scal[dd[k]] = v;
+  }
 }
 
 // scal[dst] = value for every problem of the list (one thread per entry)
@@ -398,6 +433,7 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
   if (EM == E_LOSS || EM == E_STORE_LOSS || DOT) {
     const double b = block_sum(redv, sh);
     if (threadIdx.x == 0) P.part[(int64_t)ops.region * P.pcap + lb] = b;
+    if (threadIdx.x == 0 && lb == 0) P.pn[ops.region] = nblk;
   }
 }
 
@@ -723,6 +759,7 @@ __global__ __launch_bounds__(BLOCK) void kb_smooth(const Prob *__restrict__ prob
   }
   const double b = block_sum(s, sh);
   if (threadIdx.x == 0) P.part[(int64_t)region * P.pcap + lb] = b;
+  if (threadIdx.x == 0 && lb == 0) P.pn[region] = nblk;
 }
 
 // pdot (pqdts.f90:589-614)
@@ -814,15 +851,21 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
   if (threadIdx.x == 0) P.part[(int64_t)P.pcap + lb] = t;
   t = block_sum(b1, sh);
   if (threadIdx.x == 0) P.part[2 * (int64_t)P.pcap + lb] = t;
+  if (threadIdx.x == 0 && lb == 0) P.pn[0] = P.pn[1] = P.pn[2] = nblk;
 }
 
-// CG update (pqdts.f90:2060-2070): a = b1/v2; sn += a p; r += (-a) Hp;
-// z = r / sqrt(diag); partials of r.z (region 0) and r.r (region 2).
+// CG update (pqdts.f90:2060-2070). Every block first sums the Hessian product's
+// v2 = p.Hp partials (region i_reg_v2, stored by the band GEMM), then a = b1/v2;
+// sn += a p; r += (-a) Hp; z = r / sqrt(diag); partials of r.z (region 0) and
+// r.r (region 2) for kb_cg_dir.
 __global__ __launch_bounds__(BLOCK) void kb_cg_update(const Prob *__restrict__ probs, WorkList wl, int i_b1,
-                                                      int i_v2) {
+                                                      int i_reg_v2, int i_v2) {
   __shared__ double sh[WAVES_PER_BLOCK];
+  __shared__ double bc;
   ROW_PROLOGUE();
-  const double a = P.scal[i_b1] / P.scal[i_v2];
+  const double v2 = reduce_parts_all(P.part + (int64_t)i_reg_v2 * P.pcap, P.pn[i_reg_v2], sh, &bc);
+  if (threadIdx.x == 0 && lb == 0) P.scal[i_v2] = v2;
+  const double a = P.scal[i_b1] / v2;
   double rz = 0.0, rr = 0.0;
   if (j < P.M) {
     const double sq = sqrt(full ? P.diag_full[j] : P.diag_red[j]);
@@ -842,14 +885,41 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_update(const Prob *__restrict__ p
   if (threadIdx.x == 0) P.part[lb] = t;
   t = block_sum(rr, sh);
   if (threadIdx.x == 0) P.part[2 * (int64_t)P.pcap + lb] = t;
+  if (threadIdx.x == 0 && lb == 0) P.pn[0] = P.pn[2] = nblk;
 }
 
-// New direction (pqdts.f90:2075): p = (v2/b1) p + 1 z; ev for the next BF.
+// End of CG round j (pqdts.f90:2072-2089). Every block sums the r.z and r.r
+// partials of kb_cg_update; block 0 stores them, logs (j, sqrt(r.r), eps, b1,
+// r.z) and, once sqrt(r.r) <= eps, ends this problem's CG (scal[i_act] = 0,
+// scal[i_n] = rounds done). Otherwise p = (r.z/b1) p + 1 z, and ev for the next
+// BF. A problem whose CG ends keeps its p: nothing reads it before the next
+// kb_cg_init (every block takes the same decision, whenever it starts).
 __global__ __launch_bounds__(BLOCK) void kb_cg_dir(const Prob *__restrict__ probs, WorkList wl, int i_b1,
-                                                   int i_v2new) {
+                                                   int i_rz, int i_rr, int j_round, int i_eps, int i_act,
+                                                   int i_n) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  __shared__ double bc;
   ROW_PROLOGUE();
-  if (j >= P.M) return;
-  const double beta = P.scal[i_v2new] / P.scal[i_b1];
+  const int np_ = P.pn[0];
+  const double vrz = reduce_parts_all(P.part, np_, sh, &bc);
+  const double vrr = reduce_parts_all(P.part + 2 * (int64_t)P.pcap, np_, sh, &bc);
+  double *sc = P.scal;
+  const double b2 = sqrt(vrr);
+  const bool stop = b2 <= sc[i_eps];
+  if (threadIdx.x == 0 && lb == 0) {
+    double *row = P.cglog + (int64_t)(j_round - 1) * 5;
+    row[0] = (double)j_round;
+    row[1] = b2;
+    row[2] = sc[i_eps];
+    row[3] = sc[i_b1];
+    row[4] = vrz;
+    sc[i_rz] = vrz;
+    sc[i_rr] = vrr;
+    sc[i_n] = (double)j_round;
+    if (stop) sc[i_act] = 0.0;
+  }
+  if (stop || j >= P.M) return;
+  const double beta = vrz / sc[i_b1];
   const int mr = full ? -1 : P.maxr[j];
   double psum = 0.0, own = 0.0;
   const int64_t base = (int64_t)j * N;
@@ -867,38 +937,27 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_dir(const Prob *__restrict__ prob
   }
 }
 
-// Start of the device-side CG loop: eps = min(0.5, ||g||^(1/2)) ||g|| from the
-// squared gradient norm (pqdts.f90:2028) and the per-problem "still iterating"
-// flag. One thread per work-list entry.
-__global__ void kb_cg_start(const Prob *__restrict__ probs, WorkList wl, int i_nd, int i_eps, int i_act,
-                            int i_n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= wl.np) return;
-  double *sc = probs[wl.pid[i]].scal;
-  const double nd = sc[i_nd];
-  sc[i_eps] = fmin(0.5, sqrt(sqrt(nd))) * sqrt(nd);
-  sc[i_act] = 1.0;
-  sc[i_n] = 0.0;
-}
-
-// End of CG round j (pqdts.f90:2084-2089): log (j, sqrt(b2), eps, b1, v2) and
-// leave the loop once sqrt(b2) <= eps. Problems that already left are skipped.
-__global__ void kb_cg_log(const Prob *__restrict__ probs, WorkList wl, int j, int i_b1, int i_rz, int i_rr,
-                          int i_eps, int i_act, int i_n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= wl.np) return;
-  const Prob &P = probs[wl.pid[i]];
-  double *sc = P.scal;
-  if (sc[i_act] == 0.0) return;
-  const double b2 = sqrt(sc[i_rr]);
-  double *row = P.cglog + (int64_t)(j - 1) * 5;
-  row[0] = (double)j;
-  row[1] = b2;
-  row[2] = sc[i_eps];
-  row[3] = sc[i_b1];
-  row[4] = sc[i_rz];
-  sc[i_n] = (double)j;
-  if (b2 <= sc[i_eps]) sc[i_act] = 0.0;
+// Start of PCG (pqdts.f90:2024-2031), one thread block per problem: sums of
+// kb_cg_init's partials (nact, |G|^2, r.z) into scal[i_nact], scal[i_nd],
+// scal[i_b1]; CG tolerance eps = min(0.5, sqrt(|G|)) |G|; CG marked running
+// with no rounds done.
+__global__ __launch_bounds__(BLOCK) void kb_cg_start(const Prob *__restrict__ probs, WorkList wl, int i_nact,
+                                                     int i_nd, int i_b1, int i_eps, int i_act, int i_n) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  const Prob &P = probs[wl.pid[blockIdx.x]];
+  const int n = P.pn[0];
+  const double nact = reduce_parts(P.part, n, false, sh);
+  const double nd = reduce_parts(P.part + (int64_t)P.pcap, n, false, sh);
+  const double b1 = reduce_parts(P.part + 2 * (int64_t)P.pcap, n, false, sh);
+  if (threadIdx.x == 0) {
+    double *sc = P.scal;
+    sc[i_nact] = nact;
+    sc[i_nd] = nd;
+    sc[i_b1] = b1;
+    sc[i_eps] = fmin(0.5, sqrt(sqrt(nd))) * sqrt(nd);
+    sc[i_act] = 1.0;
+    sc[i_n] = 0.0;
+  }
 }
 
 // Two-metric diagonal step (pqdts.f90:2095-2102): sn = -G/diag on active entries

@@ -110,6 +110,7 @@ struct pqdts_ctx {
   uint8_t *ik = nullptr;
   double *ev = nullptr, *gm = nullptr, *lam = nullptr, *lmin = nullptr;
   double *part = nullptr, *scal = nullptr;
+  int *pn = nullptr;  // partial counts per region (see Prob::pn)
   int pcap = 0;
   struct DBand {
     int ntiles = 0;
@@ -178,6 +179,7 @@ void refresh_desc(pqdts_ctx *c) {
   p.bg = to_band(c->bg);
   p.part = c->part;
   p.scal = c->scal;
+  p.pn = c->pn;
   HIPC(hipMemcpyAsync(c->ddesc, &c->desc, sizeof(Prob), hipMemcpyHostToDevice, c->s));
   HIPC(hipStreamSynchronize(c->s));
 }
@@ -207,19 +209,23 @@ struct Engine {
   bool sync_timers = false;
   // per-kernel HIP-event timing
   bool ktiming = false;
+  struct Work {
+    double flops = 0.0, bytes = 0.0;
+  };
   struct KPair {
     hipEvent_t a, b;
     int kid;
-    double work;
+    Work work;
     int round;                                // CG round of a guarded launch, else 0
-    std::vector<std::pair<int, double>> parts;  // guarded launch: (slot, work) per problem
+    bool idle_at_stop;                        // no work in the round where a problem's CG stops
+    std::vector<std::pair<int, Work>> parts;  // guarded launch: (slot, work) per problem
   };
   std::vector<KPair> kpending, kdeferred;
   int kround = 0;  // current CG round while CG launches are guarded
   std::vector<hipEvent_t> kpool;
   std::vector<std::string> knames;
   std::vector<int> kcount;
-  std::vector<double> kms, kwork;
+  std::vector<double> kms, kwork, kbytes;
   std::vector<std::unique_ptr<PState>> st;  // solver state per slot
 
   ~Engine();
@@ -283,6 +289,7 @@ struct Engine {
     kcount.push_back(0);
     kms.push_back(0.0);
     kwork.push_back(0.0);
+    kbytes.push_back(0.0);
     return (int)knames.size() - 1;
   }
   hipEvent_t kevent() {
@@ -301,25 +308,51 @@ struct Engine {
     HIPC(hipEventRecord(a, s));
     return a;
   }
-  // algorithmic FP64/FP32 flops of one band product of problem p: 2 x stored
-  // elements of the banded operand (F for F V and F^T W, G for G p) x N
-  double gemm_work(int p, int band) const {
-    return 2.0 * (double)(band == BAND_G ? ctx[p]->gram_nnz : ctx[p]->nspan) * ctx[p]->N;
+  // Algorithmic work of one band product of problem p: flops = 2 x stored
+  // elements of the banded operand (F for F V and F^T W, G for G p) x N; bytes =
+  // that operand once (FP32 copy in mixed precision) + the dense operand read once
+  // + the result written once (+ P for the loss and the gradient, the stencil
+  // source for the gradient, the active-set mask for the Hessian product).
+  Work gemm_work(int p, int band, bool f32 = false) const {
+    const pqdts_ctx *c = ctx[p];
+    const double MN = (double)c->MN, DN = (double)c->D * c->N, ab = f32 ? 4.0 : 8.0;
+    Work w;
+    if (band == BAND_G) {
+      w.flops = 2.0 * (double)c->gram_nnz * c->N;
+      w.bytes = ab * (double)c->gram_nnz + 16.0 * MN + MN;
+    } else if (band == BAND_F) {
+      w.flops = 2.0 * (double)c->nspan * c->N;
+      w.bytes = ab * (double)c->nspan + 8.0 * MN + 8.0 * DN;
+    } else {
+      w.flops = 2.0 * (double)c->nspan * c->N;
+      w.bytes = ab * (double)c->nspan + 16.0 * DN + 16.0 * MN + MN;
+    }
+    return w;
   }
-  // Closes a timed launch over S. Inside the guarded CG loop a problem whose CG
-  // already stopped does no work, so its share is settled by kresolve() once the
-  // round counts are known.
-  void kend(hipEvent_t a, const std::string &name, const std::vector<int> *S = nullptr, int band = -1) {
+  // Streaming kernels: `per_elem` bytes per element of the M x N arrays.
+  Work stream_work(int p, double per_elem) const {
+    Work w;
+    w.bytes = per_elem * (double)ctx[p]->MN;
+    return w;
+  }
+  // Closes a timed launch over S (work(p) per problem). Inside the guarded CG
+  // loop a problem whose CG already stopped does no work, so its share is
+  // settled by kresolve() once the round counts are known.
+  template <class WF>
+  void kend(hipEvent_t a, const std::string &name, const std::vector<int> &S, WF work,
+            bool idle_at_stop = false) {
     if (!a) return;
     hipEvent_t b = kevent();
     HIPC(hipEventRecord(b, s));
-    KPair k{a, b, kernel_id(name), 0.0, 0, {}};
-    if (S) {
-      if (guard >= 0 && kround > 0) {
-        k.round = kround;
-        for (int p : *S) k.parts.push_back({p, gemm_work(p, band)});
-      } else {
-        for (int p : *S) k.work += gemm_work(p, band);
+    KPair k{a, b, kernel_id(name), Work{}, 0, idle_at_stop, {}};
+    if (guard >= 0 && kround > 0) {
+      k.round = kround;
+      for (int p : S) k.parts.push_back({p, work(p)});
+    } else {
+      for (int p : S) {
+        const Work w = work(p);
+        k.work.flops += w.flops;
+        k.work.bytes += w.bytes;
       }
     }
     kpending.push_back(std::move(k));
@@ -330,18 +363,26 @@ struct Engine {
       HIPC(hipEventElapsedTime(&ms, p.a, p.b));
       kcount[p.kid]++;
       kms[p.kid] += ms;
-      kwork[p.kid] += p.work;
+      kwork[p.kid] += p.work.flops;
+      kbytes[p.kid] += p.work.bytes;
       kpool.push_back(p.a);
       kpool.push_back(p.b);
       if (p.round > 0) kdeferred.push_back(std::move(p));
     }
     kpending.clear();
   }
-  // Work of guarded CG launches: problem p ran rounds 1..scal[S_CGN].
+  // Work of guarded CG launches: problem p ran rounds 1..scal[S_CGN]; a launch
+  // marked idle_at_stop did nothing in the last of them if the CG stopped there.
   void kresolve() {
     for (auto &d : kdeferred)
-      for (auto &pw : d.parts)
-        if (d.round <= (int)host_scal(pw.first)[S_CGN]) kwork[d.kid] += pw.second;
+      for (auto &pw : d.parts) {
+        const double *h = host_scal(pw.first);
+        const int n = (int)h[S_CGN];
+        if (d.round < n || (d.round == n && !(d.idle_at_stop && h[S_CGACT] == 0.0))) {
+          kwork[d.kid] += pw.second.flops;
+          kbytes[d.kid] += pw.second.bytes;
+        }
+      }
     kdeferred.clear();
   }
 
@@ -414,7 +455,7 @@ struct Engine {
     launch(S, [&](int p) { return nb_gemm(p, ops.band); }, [&](const WorkList &wl, int tot) {
       kb_band_gemm<T, KC, BM, EM, MASK, DOT><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
     });
-    kend(e0, kname, &S, ops.band);
+    kend(e0, kname, S, [&](int p) { return gemm_work(p, ops.band, sizeof(T) == 4); });
   }
 
   // ================= operators (reference subroutine in brackets) =================
@@ -432,13 +473,16 @@ struct Engine {
     if (recon) reconstruct(S, x);
     GemmOps ops{BAND_F, x, OP_O, 0, 0, R_A};
     hipEvent_t e0 = kbegin();
-    launch_reduce(S, [&](int p) { return nb_gemm(p, BAND_F); }, [&](const WorkList &wl, int tot) {
+    launch(S, [&](int p) { return nb_gemm(p, BAND_F); }, [&](const WorkList &wl, int tot) {
       kb_band_gemm<double, KC, B_PLAIN, E_LOSS, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
-    }, R_A, S_LA);
-    kend(e0, "spmm_F", &S, BAND_F);
-    launch_reduce(S, [&](int p) { return std::max(0, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); },
-                  [&](const WorkList &wl, int tot) { kb_smooth<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, R_B); }, R_B,
-                  S_LB);
+    });
+    kend(e0, "spmm_F", S, [&](int p) { return gemm_work(p, BAND_F); });
+    // at least one block per problem, so that M = 1 still stores a zero partial
+    launch(S, [&](int p) { return std::max(1, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); },
+           [&](const WorkList &wl, int tot) { kb_smooth<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, R_B); });
+    launch(S, [](int) { return 1; }, [&](const WorkList &wl, int) {
+      kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 2, R_A, S_LA, R_B, S_LB, 0, 0);
+    });
   }
   double Lval(int p) { return host_scal(p)[S_LA] + host_scal(p)[S_LB]; }
 
@@ -471,10 +515,14 @@ struct Engine {
     std::vector<int> G = g64;
     G.insert(G.end(), g32.begin(), g32.end());
     std::vector<int> GR = reduced_of(G);
-    if (!GR.empty())
+    if (!GR.empty()) {
+      hipEvent_t e0 = kbegin();
       launch(GR, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
         kb_foldval_gram<<<tot, BLOCK, 0, s>>>(dprobs, wl, dd, KC, TILE);
       });
+      // a G row band and the gathered column of p~ per row
+      kend(e0, "foldval_gram", GR, [&](int p) { Work w; w.bytes = 16.0 * (double)ctx[p]->gram_nnz; return w; });
+    }
     const GemmOps hg{BAND_G, dd, bx, dd, dd, R_B};
     // B = p with its implicit columns expanded (reduced problems)
     hvp_gemm<double, B_EXPAND>(g64, hg, mask, dot, "hvp_G");
@@ -504,15 +552,16 @@ struct Engine {
     hipEvent_t e0 = kbegin();
     auto nb = [&](int p) { return nb_gemm(p, ops.band); };
     if (mask && dot) {
-      launch_reduce(S, nb, [&](const WorkList &wl, int tot) {
+      // the v2 partials (region R_B) are summed by kb_cg_update
+      launch(S, nb, [&](const WorkList &wl, int tot) {
         kb_band_gemm<T, KC, BM, E_HVP, true, true><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
-      }, R_B, S_V2);
+      });
     } else {
       launch(S, nb, [&](const WorkList &wl, int tot) {
         kb_band_gemm<T, KC, BM, E_HVP, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
       });
     }
-    kend(e0, kname, &S, ops.band);
+    kend(e0, kname, S, [&](int p) { return gemm_work(p, ops.band, sizeof(T) == 4); });
   }
 
   // pdot (pqdts.f90:589-614) -> S_DOT
@@ -530,9 +579,11 @@ struct Engine {
     bool any_full = false, any_red = false;
     for (int p : S) (ctx[p]->full ? any_full : any_red) = true;
     struct Scope sc(this, any_red && !any_full ? "doproj2" : "doproj");
+    hipEvent_t e0 = kbegin();
     launch_reduce(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
       kb_project<<<tot, BLOCK, 0, s>>>(dprobs, wl, t, form_trial ? 1 : 0, recon_again ? 1 : 0, R_C);
     }, R_C, S_XMIN, true);
+    kend(e0, "project", S, [&](int p) { return stream_work(p, form_trial ? 24.0 : 16.0); });
   }
 
   // conv_test (pqdts.f90:1034-1067): sets full space; -> S_CONV
@@ -540,9 +591,11 @@ struct Engine {
     reconstruct(S, x);
     for (int p : S) ctx[p]->full = true;
     dL(S, x, g);
+    hipEvent_t e0 = kbegin();
     launch_reduce(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
       kb_conv_rows<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, g, store ? 1 : 0, R_C);
     }, R_C, S_CONV);
+    kend(e0, "conv_rows", S, [&](int p) { return stream_work(p, store ? 24.0 : 16.0); });
   }
   double convval(int p) {
     return std::sqrt(host_scal(p)[S_CONV] / ((double)ctx[p]->N * (double)ctx[p]->M));
@@ -550,9 +603,11 @@ struct Engine {
 
   void axpby(const std::vector<int> &S, int z, int x, int y, double alpha, double beta, int kind,
              bool per_problem_alpha) {
+    hipEvent_t e0 = kbegin();
     launch(S, [&](int p) { return nb_elem(p); }, [&](const WorkList &wl, int tot) {
       kb_axpby<<<tot, BLOCK, 0, s>>>(dprobs, wl, z, x, y, alpha, beta, kind, per_problem_alpha ? 1 : 0);
     });
+    kend(e0, "axpby", S, [&](int p) { return stream_work(p, 24.0); });
   }
 
   void transform(const std::vector<int> &S, bool rebuild) {
@@ -565,23 +620,31 @@ struct Engine {
   void cg_init(const std::vector<int> &S) {
     struct Scope sc(this, "active_dLdX");
     need_maxr(S);
+    hipEvent_t e0 = kbegin();
     launch(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
       kb_cg_init<<<tot, BLOCK, 0, s>>>(dprobs, wl, 1);
-      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_A, S_NACT, 0);
-      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_B, S_ND, 0);
-      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_C, S_B1, 0);
+      kb_cg_start<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, S_NACT, S_ND, S_B1, S_EPSCG, S_CGACT, S_CGN);
     });
+    // reads X, G; writes the mask, r, z, p, sn
+    kend(e0, "cg_init", S, [&](int p) { return stream_work(p, 49.0); });
   }
-  void cg_update(const std::vector<int> &S, int i_b1, int i_rz) {
+  // CG round j: update (after H p with its v2 partials), then the direction,
+  // the round's log and the exit test
+  void cg_update(const std::vector<int> &S, int i_b1) {
+    hipEvent_t e0 = kbegin();
     launch(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
-      kb_cg_update<<<tot, BLOCK, 0, s>>>(dprobs, wl, i_b1, S_V2);
-      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_A, i_rz, 0);
-      kb_reduce<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, R_C, S_RR, 0);
+      kb_cg_update<<<tot, BLOCK, 0, s>>>(dprobs, wl, i_b1, R_B, S_V2);
     });
+    // reads sn, p, r, Hp; writes sn, r, z
+    kend(e0, "cg_update", S, [&](int p) { return stream_work(p, 56.0); });
   }
-  void cg_dir(const std::vector<int> &S, int i_b1, int i_rz) {
-    launch(S, [&](int p) { return nb_rows(p); },
-           [&](const WorkList &wl, int tot) { kb_cg_dir<<<tot, BLOCK, 0, s>>>(dprobs, wl, i_b1, i_rz); });
+  void cg_dir(const std::vector<int> &S, int i_b1, int i_rz, int j) {
+    hipEvent_t e0 = kbegin();
+    launch(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kb_cg_dir<<<tot, BLOCK, 0, s>>>(dprobs, wl, i_b1, i_rz, S_RR, j, S_EPSCG, S_CGACT, S_CGN);
+    });
+    // reads p, z; writes p
+    kend(e0, "cg_dir", S, [&](int p) { return stream_work(p, 24.0); }, true);
   }
   void twometric(const std::vector<int> &S) {
     launch(S, [&](int p) { return nb_rows(p); },
@@ -779,9 +842,6 @@ void Engine::tick() {
   dL(A, X, G);
   if (!A2.empty()) {
     cg_init(A2);  // pqdts.f90:1986-2031
-    launch(A2, [](int) { return 1; }, [&](const WorkList &wl, int) {
-      kb_cg_start<<<cdiv(wl.np, 64), 64, 0, s>>>(dprobs, wl, S_ND, S_EPSCG, S_CGACT, S_CGN);
-    });
     // PCG, pqdts.f90:2036-2090. The exit test runs on the device: every CG launch
     // skips problems whose flag dropped, and the host only checks every few rounds.
     {
@@ -792,12 +852,8 @@ void Engine::tick() {
         kround = j;
         BF(A2, XT, HP, true, true, true);
         const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
-        cg_update(A2, ib1, i_rz);
-        cg_dir(A2, ib1, i_rz);
-        const int jj = j, b1 = ib1;
-        launch(A2, [](int) { return 1; }, [&](const WorkList &wl, int) {
-          kb_cg_log<<<cdiv(wl.np, 64), 64, 0, s>>>(dprobs, wl, jj, b1, i_rz, S_RR, S_EPSCG, S_CGACT, S_CGN);
-        });
+        cg_update(A2, ib1);
+        cg_dir(A2, ib1, i_rz, j);
         ib1 = i_rz;
         if (j % 8 == 0 || j == 50) {
           fetch();
@@ -1013,7 +1069,7 @@ void destroy_ctx(pqdts_ctx *c) {
   free_band(c->bg);
   void *ptrs[] = {c->flo,       c->flen,     c->foff, c->fval, c->fhas, c->kd_lo, c->kd_hi, c->P,
                   c->O,         c->diag_full, c->diag_red, c->maxr, c->ik,  c->ev,    c->gm,    c->lam,
-                  c->lmin,      c->part,     c->scal, c->ddesc};
+                  c->lmin,      c->part,     c->scal, c->ddesc, c->pn};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (double *p : c->slot)
@@ -1125,6 +1181,8 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     c->pcap = std::max({gx * nbt, cdiv(M, WAVES_PER_BLOCK), cdiv(c->MN, EBLOCK), 1});
     c->part = dalloc<double>((int64_t)c->pcap * NREG);
     c->scal = dalloc<double>(NSCAL);
+    c->pn = dalloc<int>(NREG);
+    HIPC(hipMemsetAsync(c->pn, 0, sizeof(int) * NREG, c->s));
     c->ddesc = dalloc<Prob>(1);
     auto up = [&](void *dst, const void *src, size_t bytes) {
       if (bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->s));
@@ -1526,7 +1584,8 @@ int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double
   return rc == 0 ? n : -1;
 }
 
-static int kernel_stats_of(Engine &e, int cap, char (*names)[24], int *counts, double *ms, double *flops) {
+static int kernel_stats_of(Engine &e, int cap, char (*names)[24], int *counts, double *ms, double *flops,
+                           double *bytes) {
   HIPC(hipStreamSynchronize(e.s));
   e.kfold();
   const int n = (int)e.knames.size();
@@ -1538,6 +1597,7 @@ static int kernel_stats_of(Engine &e, int cap, char (*names)[24], int *counts, d
     if (counts) counts[i] = e.kcount[i];
     if (ms) ms[i] = e.kms[i];
     if (flops) flops[i] = e.kwork[i];
+    if (bytes) bytes[i] = e.kbytes[i];
   }
   return n;
 }
@@ -1550,6 +1610,7 @@ static void kernel_timing_of(Engine &e, int on) {
     e.kcount[i] = 0;
     e.kms[i] = 0.0;
     e.kwork[i] = 0.0;
+    e.kbytes[i] = 0.0;
   }
 }
 
@@ -1557,9 +1618,10 @@ int pqdts_kernel_timing(pqdts_ctx *ctx, int on) {
   return guarded([&] { kernel_timing_of(E1(ctx), on); });
 }
 
-int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms, double *flops) {
+int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms, double *flops,
+                       double *bytes) {
   int n = 0;
-  int rc = guarded([&] { n = kernel_stats_of(E1(ctx), cap, names, counts, ms, flops); });
+  int rc = guarded([&] { n = kernel_stats_of(E1(ctx), cap, names, counts, ms, flops, bytes); });
   return rc == 0 ? n : -1;
 }
 
@@ -1745,11 +1807,11 @@ int pqdts_batch_kernel_timing(pqdts_batch *b, int on) {
 }
 
 int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms,
-                             double *flops) {
+                             double *flops, double *bytes) {
   int n = 0;
   int rc = guarded([&] {
     require(b != nullptr, "null batch");
-    n = kernel_stats_of(b->eng, cap, names, counts, ms, flops);
+    n = kernel_stats_of(b->eng, cap, names, counts, ms, flops, bytes);
   });
   return rc == 0 ? n : -1;
 }

@@ -146,6 +146,12 @@ def config2(seed: int = 0) -> Problem:
     return Problem("cfg2", M, N, D, F, P, 1e-3, 1e-7, pi, {"eta": 0.75, "shots": 1e7, "npix": npix})
 
 
+def config3(seed: int = 0) -> Problem:
+    """M=20000, N=10000, D=40000, alpha^2 = linspace(0, 18000, D), eta=0.9, saturation at
+    N-1, 1e8 shots, gamma=1e-2, eps=1e-8 (the single large instance)."""
+    return random_problem(20000, 10000, 40000, 0.9, 1e-2, 1e8, seed, eps=1e-8, name="cfg3")
+
+
 def thinned_outcomes(alpha2: np.ndarray, eta: float, N: int) -> np.ndarray:
     """Outcome distribution of a loss (efficiency eta) + saturation (at N-1) detector
     probed by coherent states: binomial thinning of Poisson(a) photons is
@@ -238,4 +244,4 @@ def read_coo(dirname: str, name: str, nnz: int):
     return r, c, v
 
 
-CONFIGS = {"cfg1": config1, "cfg2": config2}
+CONFIGS = {"cfg1": config1, "cfg2": config2, "cfg3": config3}
