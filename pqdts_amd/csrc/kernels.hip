@@ -338,7 +338,12 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
         const int64_t idx = (int64_t)k * N + n;
         v = Bp[idx];
         if (BM == B_DIFF) v -= P.P[idx];
-        if (BM == B_EXPAND && red && n == P.maxr[k]) v = P.ev[k];
+        if (BM == B_EXPAND && red) {
+          // independent loads, selected afterwards
+          const int mr = P.maxr[k];
+          const double ek = P.ev[k];
+          if (n == mr) v = ek;
+        }
       }
       rb[i] = (T)v;
     }
@@ -396,40 +401,122 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
     __syncthreads();
   }
 
+  // Epilogue, in batches of QB outputs (q) of one MFMA tile (i, j) per lane:
+  // every operand of a batch is loaded first (indices clamped into range, values
+  // of out-of-range outputs discarded) so the loads overlap instead of each
+  // waiting on the previous output's store; then the outputs are formed in
+  // (i, j, q) order.
+  // (FP64 variants; the FP32 ones keep the one-output-at-a-time form below,
+  // which leaves them more waves per SIMD)
+  constexpr int QB = 2;
   double redv = 0.0;
   const double g2 = 2.0 * P.gamma;
+  constexpr bool GH = (EM == E_GRAD || EM == E_HVP);
+  constexpr bool LOSS = (EM == E_LOSS || EM == E_STORE_LOSS);
   const bool exp = (EM == E_HVP) && red;
-  const double *E = (EM == E_GRAD || EM == E_HVP) ? P.op[ops.e] : nullptr;
-  const double *dd = DOT ? P.op[ops.dd] : nullptr;
+  const bool sten = GH && P.gamma != 0.0;
+  const double *__restrict__ E = GH ? P.op[ops.e] : nullptr;
+  const double *__restrict__ dd = DOT ? P.op[ops.dd] : nullptr;
+  if constexpr (!FP32) {
 #pragma unroll
   for (int i = 0; i < MR; i++)
 #pragma unroll
     for (int j = 0; j < MC; j++)
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int row = r0 + wr0 + i * 16 + Mf::row(q, fk);
+      for (int q0 = 0; q0 < 4; q0 += QB) {
+        // each batch's loads come after the previous batch's stores, which bounds
+        // the registers held by loads in flight to one batch
+        __asm__ volatile("" ::: "memory");
         const int n = n0 + wc0 + j * 16 + fi;
-        if (row >= nrows || n >= N) continue;
-        const int64_t idx = (int64_t)row * N + n;
-        const double v = FP32 ? acc64[i][j][q] : (double)acc[i][j][q];
-        if (EM == E_STORE || EM == E_STORE_LOSS) Cp[idx] = v;
-        if ((EM == E_LOSS || EM == E_STORE_LOSS) && P.fhas[row]) {
-          const double t = v - P.P[idx];
-          redv = fma(t, t, redv);
-        }
-        if (EM == E_GRAD || EM == E_HVP) {
-          double g = 2.0 * v;
-          if (P.gamma != 0.0) {
-            const double e0 = stencil_src(exp, E, P.maxr, P.ev, row, n, N);
-            if (row + 1 < nrows) g = g + g2 * (e0 - stencil_src(exp, E, P.maxr, P.ev, row + 1, n, N));
-            if (row > 0) g = g - g2 * (stencil_src(exp, E, P.maxr, P.ev, row - 1, n, N) - e0);
+        const int nc = n < N ? n : N - 1;
+        int row[QB], mr0[QB];
+        double s0[QB], sp[QB], sm[QB], gmv[QB], pv[QB], dv[QB];
+        uint8_t ikv[QB], fh[QB];
+#pragma unroll
+        for (int u = 0; u < QB; u++) {
+          row[u] = r0 + wr0 + i * 16 + Mf::row(q0 + u, fk);
+          const int rc = row[u] < nrows ? row[u] : nrows - 1;
+          const int rp = rc + 1 < nrows ? rc + 1 : rc;
+          const int rm = rc > 0 ? rc - 1 : rc;
+          const int64_t idx = (int64_t)rc * N + nc;
+          if (LOSS) {
+            fh[u] = P.fhas[rc];
+            pv[u] = P.P[idx];
           }
-          if (red) g = (n == P.maxr[row]) ? 0.0 : g - P.gm[row];
-          if (MASK && P.ik[idx]) g = 0.0;
-          Cp[idx] = g;
-          if (DOT) redv = fma(dd[idx], g, redv);
+          if (GH) {
+            mr0[u] = red ? P.maxr[rc] : -1;
+            gmv[u] = red ? P.gm[rc] : 0.0;
+            if (sten) {
+              s0[u] = E[idx];
+              sp[u] = E[(int64_t)rp * N + nc];
+              sm[u] = E[(int64_t)rm * N + nc];
+              if (exp) {
+                const int mrp = P.maxr[rp], mrm = P.maxr[rm];
+                const double e0 = P.ev[rc], ep = P.ev[rp], em = P.ev[rm];
+                if (nc == mr0[u]) s0[u] = e0;
+                if (nc == mrp) sp[u] = ep;
+                if (nc == mrm) sm[u] = em;
+              }
+            }
+          }
+          if (MASK) ikv[u] = P.ik[idx];
+          if (DOT) dv[u] = dd[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < QB; u++) {
+          const int q = q0 + u;
+          if (row[u] >= nrows || n >= N) continue;
+          const int64_t idx = (int64_t)row[u] * N + n;
+          const double v = FP32 ? acc64[i][j][q] : (double)acc[i][j][q];
+          if (EM == E_STORE || EM == E_STORE_LOSS) Cp[idx] = v;
+          if (LOSS && fh[u]) {
+            const double t = v - pv[u];
+            redv = fma(t, t, redv);
+          }
+          if (GH) {
+            double g = 2.0 * v;
+            if (sten) {
+              if (row[u] + 1 < nrows) g = g + g2 * (s0[u] - sp[u]);
+              if (row[u] > 0) g = g - g2 * (sm[u] - s0[u]);
+            }
+            if (red) g = (n == mr0[u]) ? 0.0 : g - gmv[u];
+            if (MASK && ikv[u]) g = 0.0;
+            Cp[idx] = g;
+            if (DOT) redv = fma(dv[u], g, redv);
+          }
         }
       }
+  } else {
+#pragma unroll
+    for (int i = 0; i < MR; i++)
+#pragma unroll
+      for (int j = 0; j < MC; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int row = r0 + wr0 + i * 16 + Mf::row(q, fk);
+          const int n = n0 + wc0 + j * 16 + fi;
+          if (row >= nrows || n >= N) continue;
+          const int64_t idx = (int64_t)row * N + n;
+          const double v = acc64[i][j][q];
+          if (EM == E_STORE || EM == E_STORE_LOSS) Cp[idx] = v;
+          if (LOSS && P.fhas[row]) {
+            const double t = v - P.P[idx];
+            redv = fma(t, t, redv);
+          }
+          if (GH) {
+            double g = 2.0 * v;
+            if (sten) {
+              const double e0 = stencil_src(exp, E, P.maxr, P.ev, row, n, N);
+              if (row + 1 < nrows) g = g + g2 * (e0 - stencil_src(exp, E, P.maxr, P.ev, row + 1, n, N));
+              if (row > 0) g = g - g2 * (stencil_src(exp, E, P.maxr, P.ev, row - 1, n, N) - e0);
+            }
+            if (red) g = (n == P.maxr[row]) ? 0.0 : g - P.gm[row];
+            if (MASK && P.ik[idx]) g = 0.0;
+            Cp[idx] = g;
+            if (DOT) redv = fma(dd[idx], g, redv);
+          }
+        }
+  }
   if (EM == E_LOSS || EM == E_STORE_LOSS || DOT) {
     const double b = block_sum(redv, sh);
     if (threadIdx.x == 0) P.part[(int64_t)ops.region * P.pcap + lb] = b;
