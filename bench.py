@@ -215,6 +215,20 @@ def objective(ctx):
     return ctx.L("X")
 
 
+def calibration_copy(pqdts_amd, problems, ctxs):
+    """Copies of a known size (Z = X of the largest problem, 5 times) for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of a profiler pass."""
+    i = max(range(len(problems)), key=lambda k: problems[k].M * problems[k].N)
+    c = ctxs[i]
+    for _ in range(5):
+        c.pcopy("X", "Z")
+    c.get("Z")
+    nbytes = 8 * problems[i].M * problems[i].N
+    log(f"calibration copy (the last GPU work of this run): 5 x kb_axpby reading and writing {nbytes} bytes "
+        f"each (M={problems[i].M}, N={problems[i].N})")
+    return nbytes
+
+
 def workload_mix(args, pqdts_amd, workloads):
     t = time.time()
     problems = workloads.batch_mix(args.count, eps=args.eps)
@@ -231,13 +245,17 @@ def workload_mix(args, pqdts_amd, workloads):
         run_batched(pqdts_amd, problems[:8], ctxs[:8], 1, args.maxiter, cap, False, f"warmup {w}")
     mixed_runs = []
     for s in range(args.steps):
-        dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, s == 0,
+        dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, False,
                                             f"batched mixed step {s}")
         nconv = sum(1 for r, p in zip(res, problems) if r["conv"] < p.eps)
         log(f"step {s}: batched mixed {dt:.2f} s, {nconv}/{len(problems)} converged")
         mixed_runs.append((dt, lat, Xm, res, kst, nconv))
     total = sum(r[0] for r in mixed_runs)
-    dt_m, lat_m, X_m, res_m, kst_m, _ = mixed_runs[0]
+    dt_m, lat_m, X_m, res_m, _, _ = mixed_runs[0]
+    # per-kernel HIP-event timing in a run of its own (not part of `value`)
+    dt_k, _, _, _, kst_m = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, True,
+                                       "batched mixed, kernel timing")
+    log(f"kernel-timed batched mixed run: {dt_k:.2f} s")
     Lm = []
     for c, x in zip(ctxs, X_m):
         c.put("X", x)
@@ -323,6 +341,8 @@ def workload_mix(args, pqdts_amd, workloads):
                                                    args.cpu_budget, seq_t)
         except Exception as e:
             log("cpu baseline failed:", repr(e))
+    if args.calib_copy:
+        out["calibration_copy_bytes"] = calibration_copy(pqdts_amd, problems, ctxs)
     return out
 
 
@@ -549,6 +569,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--kernels", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems shard "

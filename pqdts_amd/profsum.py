@@ -3,6 +3,8 @@
   python -m pqdts_amd.profsum trace <db>                 per-kernel calls / avg / total time
   python -m pqdts_amd.profsum pmc <db> [<db> ...]         per-kernel mean counter value per dispatch
                                                           (+ VGPRs, LDS bytes, grid)
+  python -m pqdts_amd.profsum last <db> <kernel> <k>      mean counter values of the last k dispatches
+                                                          of the kernel whose name starts with <kernel>
 Kernel names are shortened to their template head.
 """
 import sqlite3
@@ -46,8 +48,27 @@ def pmc(dbs):
         print(f"{k[:58]:58s} {n:10d} " + " ".join(f"{v:16.1f}" for v in vals) + f" {vg:5d} {ag:5d} {lds:7d} {gs:9d}")
 
 
+def last(db: str, kernel: str, k: int):
+    c = sqlite3.connect(db)
+    rows = c.execute("select dispatch_id, kernel_name, counter_name, value, grid_size from counters_collection "
+                     "order by dispatch_id").fetchall()
+    per = defaultdict(dict)
+    for did, kname, cname, val, gs in rows:
+        if short(kname).startswith(kernel):
+            per[did][cname] = val
+            per[did]["grid"] = gs
+    ids = sorted(per)[-k:]
+    counters = sorted({cn for d in ids for cn in per[d] if cn != "grid"})
+    for cn in counters:
+        vals = [per[d][cn] for d in ids if cn in per[d]]
+        print(f"{kernel} last {len(ids)} dispatches (grid {per[ids[-1]]['grid']}): {cn} mean "
+              f"{sum(vals) / len(vals):.1f} per dispatch, values {', '.join(f'{v:.0f}' for v in vals)}")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         trace(sys.argv[2])
+    elif sys.argv[1] == "last":
+        last(sys.argv[2], sys.argv[3], int(sys.argv[4]))
     else:
         pmc(sys.argv[2:])
