@@ -88,6 +88,16 @@ struct WorkList {
   double a[MAXB];  // per-problem scalar argument (step length, ...)
 };
 
+// Every array a descriptor points to is in global memory, but pointers loaded from
+// memory are generic to the compiler, and generic (flat) accesses also count
+// against the LDS wait counter: a wait for an LDS operand would then wait for the
+// global loads in flight too. Accesses go through GLOBAL-qualified pointers.
+#define GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GLOBAL T *gp(T *p) {
+  return (GLOBAL T *)p;
+}
+
 __device__ __forceinline__ int64_t lmin64(int64_t a, int64_t b) { return a < b ? a : b; }
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 __device__ __forceinline__ int wave_id() { return threadIdx.x / WAVE; }
@@ -106,7 +116,7 @@ __device__ __forceinline__ int wl_find(const WorkList &wl, int bid) {
 #define BATCH_PROLOGUE()                                   \
   const int wi_ = wl_find(wl, blockIdx.x);                 \
   const Prob &P = probs[wl.pid[wi_]];                      \
-  if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;    \
+  if (wl.guard >= 0 && gp(P.scal)[wl.guard] == 0.0) return;    \
   const int lb = blockIdx.x - wl.boff[wi_];                \
   const int nblk = wl.boff[wi_ + 1] - wl.boff[wi_];        \
   const bool full = (wl.flags[wi_] & F_FULL) != 0;         \
@@ -172,7 +182,7 @@ __device__ __forceinline__ double block_min(double v, double *sh) {
 //  * inside the next kernel, each of whose blocks sums the partials itself
 //    (reduce_parts_all): no extra launch and no inter-block synchronisation.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double reduce_parts(const double *part, int n, bool is_min, double *sh) {
+__device__ __forceinline__ double reduce_parts(const GLOBAL double *part, int n, bool is_min, double *sh) {
   double v;
   if (is_min) {
     v = INFINITY;
@@ -187,7 +197,7 @@ __device__ __forceinline__ double reduce_parts(const double *part, int n, bool i
 }
 
 // reduce_parts with the result in every thread (`bc` is one shared double)
-__device__ __forceinline__ double reduce_parts_all(const double *part, int n, double *sh, double *bc) {
+__device__ __forceinline__ double reduce_parts_all(const GLOBAL double *part, int n, double *sh, double *bc) {
   const double v = reduce_parts(part, n, false, sh);
   if (threadIdx.x == 0) *bc = v;
   __syncthreads();
@@ -200,10 +210,10 @@ __global__ __launch_bounds__(BLOCK) void kb_reduce(const Prob *__restrict__ prob
                                                    int dst, int is_min) {
   __shared__ double sh[WAVES_PER_BLOCK];
   const Prob &P = probs[wl.pid[blockIdx.x]];
-  if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;
+  if (wl.guard >= 0 && gp(P.scal)[wl.guard] == 0.0) return;
   const int n = wl.boff[blockIdx.x + 1] - wl.boff[blockIdx.x];
-  const double v = reduce_parts(P.part + (int64_t)region * P.pcap, n, is_min != 0, sh);
-  if (threadIdx.x == 0) P.scal[dst] = v;
+  const double v = reduce_parts(gp(P.part) + (int64_t)region * P.pcap, n, is_min != 0, sh);
+  if (threadIdx.x == 0) gp(P.scal)[dst] = v;
 }
 
 // Sums of up to three regions (scal[d_i] = sum of pn[r_i] partials of region r_i),
@@ -212,11 +222,11 @@ __global__ __launch_bounds__(BLOCK) void kb_reduce_pn(const Prob *__restrict__ p
                                                       int d0, int r1, int d1, int r2, int d2) {
   __shared__ double sh[WAVES_PER_BLOCK];
   const Prob &P = probs[wl.pid[blockIdx.x]];
-  if (wl.guard >= 0 && P.scal[wl.guard] == 0.0) return;
+  if (wl.guard >= 0 && gp(P.scal)[wl.guard] == 0.0) return;
   const int rr[3] = {r0, r1, r2}, dd[3] = {d0, d1, d2};
   for (int k = 0; k < nr; k++) {
-    const double v = reduce_parts(P.part + (int64_t)rr[k] * P.pcap, P.pn[rr[k]], false, sh);
-    if (threadIdx.x == 0) P.scal[dd[k]] = v;
+    const double v = reduce_parts(gp(P.part) + (int64_t)rr[k] * P.pcap, gp(P.pn)[rr[k]], false, sh);
+    if (threadIdx.x == 0) gp(P.scal)[dd[k]] = v;
   }
 }
 
@@ -236,8 +246,9 @@ enum TMode {
 
 // E(j,n): the stencil source. In the reduced HVP the direction's implicit column
 // is replaced by ev[j] (the expanded vector, pqdts.f90:293-317).
-__device__ __forceinline__ double stencil_src(bool exp, const double *__restrict__ E, const int *__restrict__ maxr,
-                                              const double *__restrict__ ev, int j, int n, int N) {
+__device__ __forceinline__ double stencil_src(bool exp, const GLOBAL double *__restrict__ E,
+                                              const GLOBAL int *__restrict__ maxr, const GLOBAL double *__restrict__ ev,
+                                              int j, int n, int N) {
   if (exp && n == maxr[j]) return ev[j];
   return E[(int64_t)j * N + n];
 }
@@ -297,17 +308,22 @@ struct MfmaT<float> {
 // and every chunk's sum is added into FP64 accumulators (compensating the FP32
 // accumulation length to one chunk).
 // ---------------------------------------------------------------------------
-template <typename T, int KC, int BM, int EM, bool MASK, bool DOT>
-__global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ probs, WorkList wl, GemmOps ops) {
-  constexpr int TR = 64, TC = 64, WC = 2, MR = 2, MC = 2;
+//
+// band_gemm_tile computes output tile lb (64 x 64) of one problem with the whole
+// workgroup, LDS buffers As/Bs supplied by the caller, and adds the tile's loss or
+// v2 contributions to redv; kb_band_gemm runs one tile per workgroup over a work
+// list.
+// ---------------------------------------------------------------------------
+constexpr int GT = 64;  // rows and columns of an output tile
+
+template <typename T, int KC, int BM, int EM, bool MASK, bool DOT, bool LAT>
+__device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops, int lb, bool full,
+                                               T (*As)[KC * GT], T (*Bs)[KC * GT], double &redv) {
+  constexpr int TR = GT, TC = GT, WC = 2, MR = 2, MC = 2;
   constexpr int AE = KC * TR / BLOCK;
   constexpr int BE = KC * TC / BLOCK;
   constexpr bool FP32 = sizeof(T) == 4;
   typedef MfmaT<T> Mf;
-  __shared__ T As[2][KC * TR];
-  __shared__ T Bs[2][KC * TC];
-  __shared__ double sh[WAVES_PER_BLOCK];
-  BATCH_PROLOGUE();
   const bool red = !full;
   const Band &band = ops.band == BAND_F ? P.bf : (ops.band == BAND_FT ? P.bt : P.bg);
   const int N = P.N;
@@ -316,16 +332,16 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
   const int r0 = tile * TR;
   const int nrows = ops.band == BAND_F ? P.D : P.M;
   const int nk = ops.band == BAND_FT ? P.D : P.M;
-  const double *Bp = P.op[ops.b];
-  double *Cp = P.op[ops.c];
+  const GLOBAL double *Bp = gp(P.op[ops.b]);
+  GLOBAL double *Cp = gp(P.op[ops.c]);
   const int tid = threadIdx.x, lane = lane_id(), w = wave_id();
   const int wr0 = (w / WC) * 32, wc0 = (w % WC) * 32;
-  const T *ap = Mf::pack(band) + band.off[tile];
-  const int k0 = band.klo[tile], nch = band.nch[tile];
+  const GLOBAL T *ap = gp(Mf::pack(band)) + gp(band.off)[tile];
+  const int k0 = gp(band.klo)[tile], nch = gp(band.nch)[tile];
 
   T ra[AE], rb[BE];
   auto gload = [&](int c) {
-    const T *src = ap + (int64_t)c * (KC * TR);
+    const GLOBAL T *src = ap + (int64_t)c * (KC * TR);
 #pragma unroll
     for (int i = 0; i < AE; i++) ra[i] = src[tid + BLOCK * i];
 #pragma unroll
@@ -337,11 +353,11 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
       if (k < nk && n < N) {
         const int64_t idx = (int64_t)k * N + n;
         v = Bp[idx];
-        if (BM == B_DIFF) v -= P.P[idx];
+        if (BM == B_DIFF) v -= gp(P.P)[idx];
         if (BM == B_EXPAND && red) {
           // independent loads, selected afterwards
-          const int mr = P.maxr[k];
-          const double ek = P.ev[k];
+          const int mr = gp(P.maxr)[k];
+          const double ek = gp(P.ev)[k];
           if (n == mr) v = ek;
         }
       }
@@ -401,23 +417,22 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
     __syncthreads();
   }
 
-  // Epilogue, in batches of QB outputs (q) of one MFMA tile (i, j) per lane:
-  // every operand of a batch is loaded first (indices clamped into range, values
-  // of out-of-range outputs discarded) so the loads overlap instead of each
-  // waiting on the previous output's store; then the outputs are formed in
-  // (i, j, q) order.
-  // (FP64 variants; the FP32 ones keep the one-output-at-a-time form below,
-  // which leaves them more waves per SIMD)
+  // Epilogue. LAT (latency form, FP64, for launches too small to fill the chip):
+  // in batches of QB outputs (q) of one MFMA tile (i, j) per lane, every operand of
+  // a batch is loaded first (indices clamped into range, values of out-of-range
+  // outputs discarded) so the loads overlap instead of each waiting on the
+  // previous output's store; then the outputs are formed in (i, j, q) order.
+  // Otherwise one output at a time (fewer registers: more waves per SIMD, which
+  // hides the same latency when the launch fills the chip). Same arithmetic.
   constexpr int QB = 2;
-  double redv = 0.0;
   const double g2 = 2.0 * P.gamma;
   constexpr bool GH = (EM == E_GRAD || EM == E_HVP);
   constexpr bool LOSS = (EM == E_LOSS || EM == E_STORE_LOSS);
   const bool exp = (EM == E_HVP) && red;
   const bool sten = GH && P.gamma != 0.0;
-  const double *__restrict__ E = GH ? P.op[ops.e] : nullptr;
-  const double *__restrict__ dd = DOT ? P.op[ops.dd] : nullptr;
-  if constexpr (!FP32) {
+  const GLOBAL double *__restrict__ E = GH ? gp(P.op[ops.e]) : nullptr;
+  const GLOBAL double *__restrict__ dd = DOT ? gp(P.op[ops.dd]) : nullptr;
+  if constexpr (!FP32 && LAT) {
 #pragma unroll
   for (int i = 0; i < MR; i++)
 #pragma unroll
@@ -440,26 +455,26 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
           const int rm = rc > 0 ? rc - 1 : rc;
           const int64_t idx = (int64_t)rc * N + nc;
           if (LOSS) {
-            fh[u] = P.fhas[rc];
-            pv[u] = P.P[idx];
+            fh[u] = gp(P.fhas)[rc];
+            pv[u] = gp(P.P)[idx];
           }
           if (GH) {
-            mr0[u] = red ? P.maxr[rc] : -1;
-            gmv[u] = red ? P.gm[rc] : 0.0;
+            mr0[u] = red ? gp(P.maxr)[rc] : -1;
+            gmv[u] = red ? gp(P.gm)[rc] : 0.0;
             if (sten) {
               s0[u] = E[idx];
               sp[u] = E[(int64_t)rp * N + nc];
               sm[u] = E[(int64_t)rm * N + nc];
               if (exp) {
-                const int mrp = P.maxr[rp], mrm = P.maxr[rm];
-                const double e0 = P.ev[rc], ep = P.ev[rp], em = P.ev[rm];
+                const int mrp = gp(P.maxr)[rp], mrm = gp(P.maxr)[rm];
+                const double e0 = gp(P.ev)[rc], ep = gp(P.ev)[rp], em = gp(P.ev)[rm];
                 if (nc == mr0[u]) s0[u] = e0;
                 if (nc == mrp) sp[u] = ep;
                 if (nc == mrm) sm[u] = em;
               }
             }
           }
-          if (MASK) ikv[u] = P.ik[idx];
+          if (MASK) ikv[u] = gp(P.ik)[idx];
           if (DOT) dv[u] = dd[idx];
         }
 #pragma unroll
@@ -497,30 +512,40 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
           const int n = n0 + wc0 + j * 16 + fi;
           if (row >= nrows || n >= N) continue;
           const int64_t idx = (int64_t)row * N + n;
-          const double v = acc64[i][j][q];
+          const double v = FP32 ? acc64[i][j][q] : (double)acc[i][j][q];
           if (EM == E_STORE || EM == E_STORE_LOSS) Cp[idx] = v;
-          if (LOSS && P.fhas[row]) {
-            const double t = v - P.P[idx];
+          if (LOSS && gp(P.fhas)[row]) {
+            const double t = v - gp(P.P)[idx];
             redv = fma(t, t, redv);
           }
           if (GH) {
             double g = 2.0 * v;
             if (sten) {
-              const double e0 = stencil_src(exp, E, P.maxr, P.ev, row, n, N);
-              if (row + 1 < nrows) g = g + g2 * (e0 - stencil_src(exp, E, P.maxr, P.ev, row + 1, n, N));
-              if (row > 0) g = g - g2 * (stencil_src(exp, E, P.maxr, P.ev, row - 1, n, N) - e0);
+              const double e0 = stencil_src(exp, E, gp(P.maxr), gp(P.ev), row, n, N);
+              if (row + 1 < nrows) g = g + g2 * (e0 - stencil_src(exp, E, gp(P.maxr), gp(P.ev), row + 1, n, N));
+              if (row > 0) g = g - g2 * (stencil_src(exp, E, gp(P.maxr), gp(P.ev), row - 1, n, N) - e0);
             }
-            if (red) g = (n == P.maxr[row]) ? 0.0 : g - P.gm[row];
-            if (MASK && P.ik[idx]) g = 0.0;
+            if (red) g = (n == gp(P.maxr)[row]) ? 0.0 : g - gp(P.gm)[row];
+            if (MASK && gp(P.ik)[idx]) g = 0.0;
             Cp[idx] = g;
             if (DOT) redv = fma(dd[idx], g, redv);
           }
         }
   }
+}
+
+template <typename T, int KC, int BM, int EM, bool MASK, bool DOT, bool LAT>
+__global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ probs, WorkList wl, GemmOps ops) {
+  __shared__ T As[2][KC * GT];
+  __shared__ T Bs[2][KC * GT];
+  __shared__ double sh[WAVES_PER_BLOCK];
+  BATCH_PROLOGUE();
+  double redv = 0.0;
+  band_gemm_tile<T, KC, BM, EM, MASK, DOT, LAT>(P, ops, lb, full, As, Bs, redv);
   if (EM == E_LOSS || EM == E_STORE_LOSS || DOT) {
     const double b = block_sum(redv, sh);
-    if (threadIdx.x == 0) P.part[(int64_t)ops.region * P.pcap + lb] = b;
-    if (threadIdx.x == 0 && lb == 0) P.pn[ops.region] = nblk;
+    if (threadIdx.x == 0) gp(P.part)[(int64_t)ops.region * P.pcap + lb] = b;
+    if (threadIdx.x == 0 && lb == 0) gp(P.pn)[ops.region] = nblk;
   }
 }
 
@@ -533,16 +558,16 @@ __global__ __launch_bounds__(BLOCK) void kb_foldval(const Prob *__restrict__ pro
   const int k = lb * WAVES_PER_BLOCK + wave_id();
   if (k >= P.M) return;
   const int N = P.N;
-  const double *E = P.op[esrc];
-  const double *O = P.op[OP_O];
-  const int c = P.maxr[k];
+  const GLOBAL double *E = gp(P.op[esrc]);
+  const GLOBAL double *O = gp(P.op[OP_O]);
+  const int c = gp(P.maxr)[k];
   double acc = 0.0;
-  for (int d = P.kdlo[k] + lane_id(); d <= P.kdhi[k]; d += WAVE) {
-    const int rel = k - P.flo[d];
-    if (rel >= 0 && rel < P.flen[d]) {
+  for (int d = gp(P.kdlo)[k] + lane_id(); d <= gp(P.kdhi)[k]; d += WAVE) {
+    const int rel = k - gp(P.flo)[d];
+    if (rel >= 0 && rel < gp(P.flen)[d]) {
       double wv = O[(int64_t)d * N + c];
-      if (MODE == T_GRAD) wv -= P.P[(int64_t)d * N + c];
-      acc = fma(P.fval[P.foff[d] + rel], wv, acc);
+      if (MODE == T_GRAD) wv -= gp(P.P)[(int64_t)d * N + c];
+      acc = fma(gp(P.fval)[gp(P.foff)[d] + rel], wv, acc);
     }
   }
   acc = wave_sum(acc);
@@ -551,46 +576,50 @@ __global__ __launch_bounds__(BLOCK) void kb_foldval(const Prob *__restrict__ pro
     double g = 2.0 * acc;
     if (P.gamma != 0.0) {
       const double g2 = 2.0 * P.gamma;
-      const double e0 = stencil_src(exp, E, P.maxr, P.ev, k, c, N);
-      if (k + 1 < P.M) g = g + g2 * (e0 - stencil_src(exp, E, P.maxr, P.ev, k + 1, c, N));
-      if (k > 0) g = g - g2 * (stencil_src(exp, E, P.maxr, P.ev, k - 1, c, N) - e0);
+      const double e0 = stencil_src(exp, E, gp(P.maxr), gp(P.ev), k, c, N);
+      if (k + 1 < P.M) g = g + g2 * (e0 - stencil_src(exp, E, gp(P.maxr), gp(P.ev), k + 1, c, N));
+      if (k > 0) g = g - g2 * (stencil_src(exp, E, gp(P.maxr), gp(P.ev), k - 1, c, N) - e0);
     }
-    P.gm[k] = g;
+    gp(P.gm)[k] = g;
   }
 }
 
 // Gram path of BF: pre-fold value (2 G p~ + stencil)[k, maxr[k]] from the packed
 // band of G = F^T F, one wave per k; p~ is dd with its implicit column replaced
 // by ev. Reduced-space problems only.
-__global__ __launch_bounds__(BLOCK) void kb_foldval_gram(const Prob *__restrict__ probs, WorkList wl, int ddsrc,
-                                                         int kc, int tile) {
-  BATCH_PROLOGUE();
-  const int k = lb * WAVES_PER_BLOCK + wave_id();
-  if (k >= P.M) return;
+__device__ __forceinline__ void foldval_gram_row(const Prob &P, const GLOBAL double *__restrict__ dd, int k, int kc,
+                                                 int tile) {
   const int N = P.N, M = P.M;
-  const double *dd = P.op[ddsrc];
-  const int c = P.maxr[k];
+  const int c = gp(P.maxr)[k];
   const int t = k / tile, r = k - t * tile;
-  const int l0 = P.bg.klo[t], nl = P.bg.nch[t] * kc;
-  const double *gp = P.bg.pack + P.bg.off[t] + r;
+  const int l0 = gp(P.bg.klo)[t], nl = gp(P.bg.nch)[t] * kc;
+  const GLOBAL double *gpk = gp(P.bg.pack) + gp(P.bg.off)[t] + r;
   double acc = 0.0;
   for (int j = lane_id(); j < nl; j += WAVE) {
     const int l = l0 + j;
     if (l >= M) break;
-    const double p = (c == P.maxr[l]) ? P.ev[l] : dd[(int64_t)l * N + c];
-    acc = fma(gp[(int64_t)j * tile], p, acc);
+    const double p = (c == gp(P.maxr)[l]) ? gp(P.ev)[l] : dd[(int64_t)l * N + c];
+    acc = fma(gpk[(int64_t)j * tile], p, acc);
   }
   acc = wave_sum(acc);
   if (lane_id() == 0) {
     double g = 2.0 * acc;
     if (P.gamma != 0.0) {
       const double g2 = 2.0 * P.gamma;
-      const double e0 = stencil_src(true, dd, P.maxr, P.ev, k, c, N);
-      if (k + 1 < M) g = g + g2 * (e0 - stencil_src(true, dd, P.maxr, P.ev, k + 1, c, N));
-      if (k > 0) g = g - g2 * (stencil_src(true, dd, P.maxr, P.ev, k - 1, c, N) - e0);
+      const double e0 = stencil_src(true, dd, gp(P.maxr), gp(P.ev), k, c, N);
+      if (k + 1 < M) g = g + g2 * (e0 - stencil_src(true, dd, gp(P.maxr), gp(P.ev), k + 1, c, N));
+      if (k > 0) g = g - g2 * (stencil_src(true, dd, gp(P.maxr), gp(P.ev), k - 1, c, N) - e0);
     }
-    P.gm[k] = g;
+    gp(P.gm)[k] = g;
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_foldval_gram(const Prob *__restrict__ probs, WorkList wl, int ddsrc,
+                                                         int kc, int tile) {
+  BATCH_PROLOGUE();
+  const int k = lb * WAVES_PER_BLOCK + wave_id();
+  if (k >= P.M) return;
+  foldval_gram_row(P, gp(P.op[ddsrc]), k, kc, tile);
 }
 
 // Dense copy of F (D x M, row-major) into `out`, one wave per row: used once to
@@ -600,10 +629,10 @@ __global__ __launch_bounds__(BLOCK) void kb_dense_rows(const Prob *__restrict__ 
   BATCH_PROLOGUE();
   const int d = lb * WAVES_PER_BLOCK + wave_id();
   if (d >= P.D) return;
-  double *row = out + (int64_t)d * P.M;
+  GLOBAL double *row = gp(out) + (int64_t)d * P.M;
   for (int k = lane_id(); k < P.M; k += WAVE) {
-    const int rel = k - P.flo[d];
-    row[k] = (rel >= 0 && rel < P.flen[d]) ? P.fval[P.foff[d] + rel] : 0.0;
+    const int rel = k - gp(P.flo)[d];
+    row[k] = (rel >= 0 && rel < gp(P.flen)[d]) ? gp(P.fval)[gp(P.foff)[d] + rel] : 0.0;
   }
 }
 
@@ -614,10 +643,10 @@ __global__ __launch_bounds__(BLOCK) void kb_bdiag(const Prob *__restrict__ probs
   const int k = lb * WAVES_PER_BLOCK + wave_id();
   if (k >= P.M) return;
   double acc = 0.0;
-  for (int d = P.kdlo[k] + lane_id(); d <= P.kdhi[k]; d += WAVE) {
-    const int rel = k - P.flo[d];
-    if (rel >= 0 && rel < P.flen[d]) {
-      const double f = P.fval[P.foff[d] + rel];
+  for (int d = gp(P.kdlo)[k] + lane_id(); d <= gp(P.kdhi)[k]; d += WAVE) {
+    const int rel = k - gp(P.flo)[d];
+    if (rel >= 0 && rel < gp(P.flen)[d]) {
+      const double f = gp(P.fval)[gp(P.foff)[d] + rel];
       acc += 2.0 * (f * f);
     }
   }
@@ -627,8 +656,8 @@ __global__ __launch_bounds__(BLOCK) void kb_bdiag(const Prob *__restrict__ probs
       if (k + 1 < P.M) acc += 2.0 * P.gamma;
       if (k > 0) acc += 2.0 * P.gamma;
     }
-    const_cast<double *>(P.diag_full)[k] = acc;
-    const_cast<double *>(P.diag_red)[k] = acc * 2.0;
+    gp(const_cast<double *>(P.diag_full))[k] = acc;
+    gp(const_cast<double *>(P.diag_red))[k] = acc * 2.0;
   }
 }
 
@@ -642,7 +671,7 @@ __global__ __launch_bounds__(BLOCK) void kb_bdiag(const Prob *__restrict__ probs
   const int N = P.N;                                  \
   (void)N
 
-__device__ __forceinline__ double row_sum(const double *__restrict__ row, int N) {
+__device__ __forceinline__ double row_sum(const GLOBAL double *__restrict__ row, int N) {
   double s = 0.0;
   for (int n = lane_id(); n < N; n += WAVE) s += row[n];
   return wave_sum(s);
@@ -652,10 +681,10 @@ __device__ __forceinline__ double row_sum(const double *__restrict__ row, int N)
 __global__ __launch_bounds__(BLOCK) void kb_reconstruct(const Prob *__restrict__ probs, WorkList wl, int x) {
   ROW_PROLOGUE();
   if (j >= P.M || full) return;
-  double *row = P.op[x] + (int64_t)j * N;
+  GLOBAL double *row = gp(P.op[x]) + (int64_t)j * N;
   const double s = row_sum(row, N);
   if (lane_id() == 0) {
-    const int r = P.maxr[j];
+    const int r = gp(P.maxr)[j];
     row[r] = 1.0 - (s - row[r]);
   }
 }
@@ -664,9 +693,9 @@ __global__ __launch_bounds__(BLOCK) void kb_reconstruct(const Prob *__restrict__
 __global__ __launch_bounds__(BLOCK) void kb_others(const Prob *__restrict__ probs, WorkList wl, int v) {
   ROW_PROLOGUE();
   if (j >= P.M || full) return;
-  const double *row = P.op[v] + (int64_t)j * N;
+  const GLOBAL double *row = gp(P.op[v]) + (int64_t)j * N;
   const double s = row_sum(row, N);
-  if (lane_id() == 0) P.ev[j] = -(s - row[P.maxr[j]]);
+  if (lane_id() == 0) gp(P.ev)[j] = -(s - row[gp(P.maxr)[j]]);
 }
 
 // Reduced-space transform of X (pqdts.f90:1970-1980; mode 1 without the first
@@ -677,11 +706,11 @@ __global__ __launch_bounds__(BLOCK) void kb_transform(const Prob *__restrict__ p
   ROW_PROLOGUE();
   if (j >= P.M) return;
   if (full) {
-    if (lane_id() == 0) P.maxr[j] = -1;
+    if (lane_id() == 0) gp(P.maxr)[j] = -1;
     return;
   }
-  double *row = P.op[0] + (int64_t)j * N;
-  const int old = P.maxr[j];
+  GLOBAL double *row = gp(P.op[0]) + (int64_t)j * N;
+  const int old = gp(P.maxr)[j];
   const bool rb = rebuild && old >= 0 && old < N;
   double newv = 0.0;
   if (rb) {
@@ -701,7 +730,7 @@ __global__ __launch_bounds__(BLOCK) void kb_transform(const Prob *__restrict__ p
   if (lane_id() == 0) {
     if (rb) row[old] = newv;
     row[bi] = 1.0;
-    P.maxr[j] = bi;
+    gp(P.maxr)[j] = bi;
   }
 }
 
@@ -721,9 +750,9 @@ __global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ pro
   ROW_PROLOGUE();
   double xm = 0.0;
   if (j < P.M) {
-    double *row = P.op[t] + (int64_t)j * N;
-    const double *xr = P.op[0] + (int64_t)j * N;
-    const double *sr = P.op[4] + (int64_t)j * N;
+    GLOBAL double *row = gp(P.op[t]) + (int64_t)j * N;
+    const GLOBAL double *xr = gp(P.op[0]) + (int64_t)j * N;
+    const GLOBAL double *sr = gp(P.op[4]) + (int64_t)j * N;
     if (full) {
       double s = 0.0;
       for (int n = lane_id(); n < N; n += WAVE) {
@@ -761,7 +790,7 @@ __global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ pro
         row[n] = v > tau ? v - tau : 0.0;
       }
     } else {
-      const int r = P.maxr[j];
+      const int r = gp(P.maxr)[j];
       double s = 0.0, own = 0.0;
       for (int n = lane_id(); n < N; n += WAVE) {
         double v = form_trial ? arg * sr[n] + 1.0 * xr[n] : row[n];
@@ -789,7 +818,7 @@ __global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ pro
   if (threadIdx.x == 0) {
     double v = sh[0];
     for (int w = 1; w < WAVES_PER_BLOCK; w++) v = fmin(v, sh[w]);
-    P.part[(int64_t)region * P.pcap + lb] = v;
+    gp(P.part)[(int64_t)region * P.pcap + lb] = v;
   }
 }
 
@@ -801,7 +830,7 @@ __global__ __launch_bounds__(BLOCK) void kb_conv_rows(const Prob *__restrict__ p
   ROW_PROLOGUE();
   double s = 0.0;
   if (j < P.M) {
-    const double *gr = P.op[g] + (int64_t)j * N, *xr = P.op[x] + (int64_t)j * N;
+    const GLOBAL double *gr = gp(P.op[g]) + (int64_t)j * N, *xr = gp(P.op[x]) + (int64_t)j * N;
     double mn = INFINITY;
     for (int n = lane_id(); n < N; n += WAVE) mn = fmin(mn, gr[n]);
     mn = wave_min(mn);
@@ -815,13 +844,13 @@ __global__ __launch_bounds__(BLOCK) void kb_conv_rows(const Prob *__restrict__ p
       for (int n = lane_id(); n < N; n += WAVE) m2 = fmin(m2, gr[n] + lambda);
       m2 = wave_min(m2);
       if (lane_id() == 0) {
-        P.lam[j] = lambda;
-        P.lmin[j] = m2;
+        gp(P.lam)[j] = lambda;
+        gp(P.lmin)[j] = m2;
       }
     }
   }
   const double b = block_sum(s, sh);
-  if (threadIdx.x == 0) P.part[(int64_t)region * P.pcap + lb] = b;
+  if (threadIdx.x == 0) gp(P.part)[(int64_t)region * P.pcap + lb] = b;
 }
 
 // ---------------------------------------------------------------------------
@@ -837,7 +866,7 @@ __global__ __launch_bounds__(BLOCK) void kb_smooth(const Prob *__restrict__ prob
   __shared__ double sh[WAVES_PER_BLOCK];
   BATCH_PROLOGUE();
   const int64_t total = (int64_t)(P.M - 1) * P.N;
-  const double *X = P.op[x];
+  const GLOBAL double *X = gp(P.op[x]);
   double s = 0.0;
   for (int64_t e = (int64_t)lb * EBLOCK + threadIdx.x; e < lmin64(total, (int64_t)(lb + 1) * EBLOCK);
        e += BLOCK) {
@@ -845,8 +874,8 @@ __global__ __launch_bounds__(BLOCK) void kb_smooth(const Prob *__restrict__ prob
     s += P.gamma * (t * t);
   }
   const double b = block_sum(s, sh);
-  if (threadIdx.x == 0) P.part[(int64_t)region * P.pcap + lb] = b;
-  if (threadIdx.x == 0 && lb == 0) P.pn[region] = nblk;
+  if (threadIdx.x == 0) gp(P.part)[(int64_t)region * P.pcap + lb] = b;
+  if (threadIdx.x == 0 && lb == 0) gp(P.pn)[region] = nblk;
 }
 
 // pdot (pqdts.f90:589-614)
@@ -855,13 +884,13 @@ __global__ __launch_bounds__(BLOCK) void kb_dot(const Prob *__restrict__ probs, 
   __shared__ double sh[WAVES_PER_BLOCK];
   BATCH_PROLOGUE();
   const int64_t total = (int64_t)P.M * P.N;
-  const double *xa = P.op[x], *ya = P.op[y];
+  const GLOBAL double *xa = gp(P.op[x]), *ya = gp(P.op[y]);
   double s = 0.0;
   for (int64_t e = (int64_t)lb * EBLOCK + threadIdx.x; e < lmin64(total, (int64_t)(lb + 1) * EBLOCK);
        e += BLOCK)
     s = fma(xa[e], ya[e], s);
   const double b = block_sum(s, sh);
-  if (threadIdx.x == 0) P.part[(int64_t)region * P.pcap + lb] = b;
+  if (threadIdx.x == 0) gp(P.part)[(int64_t)region * P.pcap + lb] = b;
 }
 
 // BLAS-1 of the reference (pqdts.f90:485-587): z = beta*y + alpha*x and its
@@ -872,8 +901,8 @@ __global__ __launch_bounds__(BLOCK) void kb_axpby(const Prob *__restrict__ probs
   BATCH_PROLOGUE();
   const int64_t total = (int64_t)P.M * P.N;
   const double al = per_problem_alpha ? arg : alpha;
-  double *za = P.op[z];
-  const double *xa = x >= 0 ? P.op[x] : nullptr, *ya = y >= 0 ? P.op[y] : nullptr;
+  GLOBAL double *za = gp(P.op[z]);
+  const GLOBAL double *xa = x >= 0 ? gp(P.op[x]) : nullptr, *ya = y >= 0 ? gp(P.op[y]) : nullptr;
   for (int64_t e = (int64_t)lb * EBLOCK + threadIdx.x; e < lmin64(total, (int64_t)(lb + 1) * EBLOCK);
        e += BLOCK) {
     double v;
@@ -901,27 +930,27 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
   ROW_PROLOGUE();
   double nact = 0.0, nd = 0.0, b1 = 0.0;
   if (j < P.M) {
-    const double sq = sqrt(full ? P.diag_full[j] : P.diag_red[j]);
-    const int mr = full ? -1 : P.maxr[j];
+    const double sq = sqrt(full ? gp(P.diag_full)[j] : gp(P.diag_red)[j]);
+    const int mr = full ? -1 : gp(P.maxr)[j];
     double psum = 0.0, own = 0.0;
     const int64_t base = (int64_t)j * N;
     for (int n = lane_id(); n < N; n += WAVE) {
       const int64_t idx = base + n;
-      double g = P.op[1][idx];
+      double g = gp(P.op[1])[idx];
       uint8_t a = 0;
-      if (tactive && P.op[0][idx] < kActiveTol && g > 0.0) {
+      if (tactive && gp(P.op[0])[idx] < kActiveTol && g > 0.0) {
         a = 1;
         g = 0.0;
         nact += 1.0;
       }
-      P.ik[idx] = a;
+      gp(P.ik)[idx] = a;
       nd = fma(g, g, nd);
       const double r = g * -1.0;
       const double zz = r / sq;
-      P.op[1][idx] = r;
-      P.op[5][idx] = zz;
-      P.op[3][idx] = zz;
-      P.op[4][idx] = 0.0;
+      gp(P.op[1])[idx] = r;
+      gp(P.op[5])[idx] = zz;
+      gp(P.op[3])[idx] = zz;
+      gp(P.op[4])[idx] = 0.0;
       b1 = fma(r, zz, b1);
       psum += zz;
       if (n == mr) own = zz;
@@ -929,16 +958,55 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
     if (!full) {
       psum = wave_sum(psum);
       own = wave_sum(own);
-      if (lane_id() == 0) P.ev[j] = -(psum - own);
+      if (lane_id() == 0) gp(P.ev)[j] = -(psum - own);
     }
   }
   double t = block_sum(nact, sh);
-  if (threadIdx.x == 0) P.part[lb] = t;
+  if (threadIdx.x == 0) gp(P.part)[lb] = t;
   t = block_sum(nd, sh);
-  if (threadIdx.x == 0) P.part[(int64_t)P.pcap + lb] = t;
+  if (threadIdx.x == 0) gp(P.part)[(int64_t)P.pcap + lb] = t;
   t = block_sum(b1, sh);
-  if (threadIdx.x == 0) P.part[2 * (int64_t)P.pcap + lb] = t;
-  if (threadIdx.x == 0 && lb == 0) P.pn[0] = P.pn[1] = P.pn[2] = nblk;
+  if (threadIdx.x == 0) gp(P.part)[2 * (int64_t)P.pcap + lb] = t;
+  if (threadIdx.x == 0 && lb == 0) gp(P.pn)[0] = gp(P.pn)[1] = gp(P.pn)[2] = nblk;
+}
+
+// One row j of the CG update, one wave: sn += a p; r += (-a) Hp; z = r / sqrt(diag);
+// r.z and r.r accumulate into rz, rr (pqdts.f90:2060-2070).
+__device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, double a, double &rz, double &rr) {
+  const int N = P.N;
+  const double sq = sqrt(full ? gp(P.diag_full)[j] : gp(P.diag_red)[j]);
+  const int64_t base = (int64_t)j * N;
+  for (int n = lane_id(); n < N; n += WAVE) {
+    const int64_t idx = base + n;
+    gp(P.op[4])[idx] = gp(P.op[4])[idx] + a * gp(P.op[3])[idx];
+    const double r = gp(P.op[1])[idx] + (-a) * gp(P.op[2])[idx];
+    gp(P.op[1])[idx] = r;
+    const double zz = r / sq;
+    gp(P.op[5])[idx] = zz;
+    rz = fma(r, zz, rz);
+    rr = fma(r, r, rr);
+  }
+}
+
+// One row j of the new direction, one wave: p = beta p + 1 z (pqdts.f90:2075) and,
+// in reduced space, ev[j] = -(sum p[j,:] - p[j,maxr]) for the next product.
+__device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, double beta) {
+  const int N = P.N;
+  const int mr = full ? -1 : gp(P.maxr)[j];
+  double psum = 0.0, own = 0.0;
+  const int64_t base = (int64_t)j * N;
+  for (int n = lane_id(); n < N; n += WAVE) {
+    const int64_t idx = base + n;
+    const double p = beta * gp(P.op[3])[idx] + 1.0 * gp(P.op[5])[idx];
+    gp(P.op[3])[idx] = p;
+    psum += p;
+    if (n == mr) own = p;
+  }
+  if (!full) {
+    psum = wave_sum(psum);
+    own = wave_sum(own);
+    if (lane_id() == 0) gp(P.ev)[j] = -(psum - own);
+  }
 }
 
 // CG update (pqdts.f90:2060-2070). Every block first sums the Hessian product's
@@ -950,29 +1018,16 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_update(const Prob *__restrict__ p
   __shared__ double sh[WAVES_PER_BLOCK];
   __shared__ double bc;
   ROW_PROLOGUE();
-  const double v2 = reduce_parts_all(P.part + (int64_t)i_reg_v2 * P.pcap, P.pn[i_reg_v2], sh, &bc);
-  if (threadIdx.x == 0 && lb == 0) P.scal[i_v2] = v2;
-  const double a = P.scal[i_b1] / v2;
+  const double v2 = reduce_parts_all(gp(P.part) + (int64_t)i_reg_v2 * P.pcap, gp(P.pn)[i_reg_v2], sh, &bc);
+  if (threadIdx.x == 0 && lb == 0) gp(P.scal)[i_v2] = v2;
+  const double a = gp(P.scal)[i_b1] / v2;
   double rz = 0.0, rr = 0.0;
-  if (j < P.M) {
-    const double sq = sqrt(full ? P.diag_full[j] : P.diag_red[j]);
-    const int64_t base = (int64_t)j * N;
-    for (int n = lane_id(); n < N; n += WAVE) {
-      const int64_t idx = base + n;
-      P.op[4][idx] = P.op[4][idx] + a * P.op[3][idx];
-      const double r = P.op[1][idx] + (-a) * P.op[2][idx];
-      P.op[1][idx] = r;
-      const double zz = r / sq;
-      P.op[5][idx] = zz;
-      rz = fma(r, zz, rz);
-      rr = fma(r, r, rr);
-    }
-  }
+  if (j < P.M) cg_update_row(P, j, full, a, rz, rr);
   double t = block_sum(rz, sh);
-  if (threadIdx.x == 0) P.part[lb] = t;
+  if (threadIdx.x == 0) gp(P.part)[lb] = t;
   t = block_sum(rr, sh);
-  if (threadIdx.x == 0) P.part[2 * (int64_t)P.pcap + lb] = t;
-  if (threadIdx.x == 0 && lb == 0) P.pn[0] = P.pn[2] = nblk;
+  if (threadIdx.x == 0) gp(P.part)[2 * (int64_t)P.pcap + lb] = t;
+  if (threadIdx.x == 0 && lb == 0) gp(P.pn)[0] = gp(P.pn)[2] = nblk;
 }
 
 // End of CG round j (pqdts.f90:2072-2089). Every block sums the r.z and r.r
@@ -987,14 +1042,14 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_dir(const Prob *__restrict__ prob
   __shared__ double sh[WAVES_PER_BLOCK];
   __shared__ double bc;
   ROW_PROLOGUE();
-  const int np_ = P.pn[0];
-  const double vrz = reduce_parts_all(P.part, np_, sh, &bc);
-  const double vrr = reduce_parts_all(P.part + 2 * (int64_t)P.pcap, np_, sh, &bc);
-  double *sc = P.scal;
+  const int np_ = gp(P.pn)[0];
+  const double vrz = reduce_parts_all(gp(P.part), np_, sh, &bc);
+  const double vrr = reduce_parts_all(gp(P.part) + 2 * (int64_t)P.pcap, np_, sh, &bc);
+  GLOBAL double *sc = gp(P.scal);
   const double b2 = sqrt(vrr);
   const bool stop = b2 <= sc[i_eps];
   if (threadIdx.x == 0 && lb == 0) {
-    double *row = P.cglog + (int64_t)(j_round - 1) * 5;
+    GLOBAL double *row = gp(P.cglog) + (int64_t)(j_round - 1) * 5;
     row[0] = (double)j_round;
     row[1] = b2;
     row[2] = sc[i_eps];
@@ -1006,22 +1061,7 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_dir(const Prob *__restrict__ prob
     if (stop) sc[i_act] = 0.0;
   }
   if (stop || j >= P.M) return;
-  const double beta = vrz / sc[i_b1];
-  const int mr = full ? -1 : P.maxr[j];
-  double psum = 0.0, own = 0.0;
-  const int64_t base = (int64_t)j * N;
-  for (int n = lane_id(); n < N; n += WAVE) {
-    const int64_t idx = base + n;
-    const double p = beta * P.op[3][idx] + 1.0 * P.op[5][idx];
-    P.op[3][idx] = p;
-    psum += p;
-    if (n == mr) own = p;
-  }
-  if (!full) {
-    psum = wave_sum(psum);
-    own = wave_sum(own);
-    if (lane_id() == 0) P.ev[j] = -(psum - own);
-  }
+  cg_dir_row(P, j, full, vrz / sc[i_b1]);
 }
 
 // Start of PCG (pqdts.f90:2024-2031), one thread block per problem: sums of
@@ -1032,12 +1072,12 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_start(const Prob *__restrict__ pr
                                                      int i_nd, int i_b1, int i_eps, int i_act, int i_n) {
   __shared__ double sh[WAVES_PER_BLOCK];
   const Prob &P = probs[wl.pid[blockIdx.x]];
-  const int n = P.pn[0];
-  const double nact = reduce_parts(P.part, n, false, sh);
-  const double nd = reduce_parts(P.part + (int64_t)P.pcap, n, false, sh);
-  const double b1 = reduce_parts(P.part + 2 * (int64_t)P.pcap, n, false, sh);
+  const int n = gp(P.pn)[0];
+  const double nact = reduce_parts(gp(P.part), n, false, sh);
+  const double nd = reduce_parts(gp(P.part) + (int64_t)P.pcap, n, false, sh);
+  const double b1 = reduce_parts(gp(P.part) + 2 * (int64_t)P.pcap, n, false, sh);
   if (threadIdx.x == 0) {
-    double *sc = P.scal;
+    GLOBAL double *sc = gp(P.scal);
     sc[i_nact] = nact;
     sc[i_nd] = nd;
     sc[i_b1] = b1;
@@ -1052,12 +1092,12 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_start(const Prob *__restrict__ pr
 __global__ __launch_bounds__(BLOCK) void kb_twometric(const Prob *__restrict__ probs, WorkList wl) {
   ROW_PROLOGUE();
   if (j >= P.M) return;
-  const double dj = full ? P.diag_full[j] : P.diag_red[j];
-  const int mr = full ? -1 : P.maxr[j];
+  const double dj = full ? gp(P.diag_full)[j] : gp(P.diag_red)[j];
+  const int mr = full ? -1 : gp(P.maxr)[j];
   const int64_t base = (int64_t)j * N;
   for (int n = lane_id(); n < N; n += WAVE) {
     const int64_t idx = base + n;
-    if (P.ik[idx] == 1 && mr != n) P.op[4][idx] = -P.op[1][idx] / dj;
+    if (gp(P.ik)[idx] == 1 && mr != n) gp(P.op[4])[idx] = -gp(P.op[1])[idx] / dj;
   }
 }
 

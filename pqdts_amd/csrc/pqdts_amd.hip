@@ -72,6 +72,22 @@ enum { R_A = 0, R_B = 1, R_C = 2 };
 
 inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// Band GEMM launch. FP64 launches with fewer workgroups than fill the chip (4 per
+// CU) take the latency form of the epilogue, larger ones the occupancy form (the
+// two measured 4-13 % apart per CG round, in opposite directions, in the two
+// regimes). FP32 has one form.
+constexpr int LAT_BLOCKS = 1024;
+template <typename T, int BM, int EM, bool MASK, bool DOT>
+void launch_gemm(int tot, hipStream_t s, const Prob *probs, const WorkList &wl, const GemmOps &ops) {
+  if constexpr (sizeof(T) == 8) {
+    if (tot < LAT_BLOCKS) {
+      kb_band_gemm<T, KC, BM, EM, MASK, DOT, true><<<tot, BLOCK, 0, s>>>(probs, wl, ops);
+      return;
+    }
+  }
+  kb_band_gemm<T, KC, BM, EM, MASK, DOT, false><<<tot, BLOCK, 0, s>>>(probs, wl, ops);
+}
+
 template <class T>
 T *dalloc(int64_t n) {
   void *p = nullptr;
@@ -453,7 +469,7 @@ struct Engine {
   void gemm_t(const std::vector<int> &S, GemmOps ops, const char *kname) {
     hipEvent_t e0 = kbegin();
     launch(S, [&](int p) { return nb_gemm(p, ops.band); }, [&](const WorkList &wl, int tot) {
-      kb_band_gemm<T, KC, BM, EM, MASK, DOT><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
+      launch_gemm<T, BM, EM, MASK, DOT>(tot, s, dprobs, wl, ops);
     });
     kend(e0, kname, S, [&](int p) { return gemm_work(p, ops.band, sizeof(T) == 4); });
   }
@@ -474,7 +490,7 @@ struct Engine {
     GemmOps ops{BAND_F, x, OP_O, 0, 0, R_A};
     hipEvent_t e0 = kbegin();
     launch(S, [&](int p) { return nb_gemm(p, BAND_F); }, [&](const WorkList &wl, int tot) {
-      kb_band_gemm<double, KC, B_PLAIN, E_LOSS, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
+      launch_gemm<double, B_PLAIN, E_LOSS, false, false>(tot, s, dprobs, wl, ops);
     });
     kend(e0, "spmm_F", S, [&](int p) { return gemm_work(p, BAND_F); });
     // at least one block per problem, so that M = 1 still stores a zero partial
@@ -554,11 +570,11 @@ struct Engine {
     if (mask && dot) {
       // the v2 partials (region R_B) are summed by kb_cg_update
       launch(S, nb, [&](const WorkList &wl, int tot) {
-        kb_band_gemm<T, KC, BM, E_HVP, true, true><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
+        launch_gemm<T, BM, E_HVP, true, true>(tot, s, dprobs, wl, ops);
       });
     } else {
       launch(S, nb, [&](const WorkList &wl, int tot) {
-        kb_band_gemm<T, KC, BM, E_HVP, false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, ops);
+        launch_gemm<T, BM, E_HVP, false, false>(tot, s, dprobs, wl, ops);
       });
     }
     kend(e0, kname, S, [&](int p) { return gemm_work(p, ops.band, sizeof(T) == 4); });
@@ -1038,7 +1054,7 @@ void ensure_gram(pqdts_ctx *c) {
     wl.boff[1] = cdiv(M, TILE) * c->bt.ntiles;
     // G = F^T F: band F^T (rows k, columns d), B = dense F (D x M) with N := M
     GemmOps ops{BAND_FT, 0, 1, 0, 0, 0};
-    kb_band_gemm<double, KC, B_PLAIN, E_STORE, false, false><<<wl.boff[1], BLOCK, 0, c->s>>>(dd, wl, ops);
+    launch_gemm<double, B_PLAIN, E_STORE, false, false>(wl.boff[1], c->s, dd, wl, ops);
     HIPC(hipGetLastError());
     std::vector<double> Gh((size_t)M * M);
     HIPC(hipMemcpyAsync(Gh.data(), Gd, sizeof(double) * Gh.size(), hipMemcpyDeviceToHost, c->s));
