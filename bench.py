@@ -178,36 +178,60 @@ def cpu_baseline_cfg2(prob, op_counts, threads):
 
 
 # ------------------------------------------------------------------ workloads
-def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing, tag):
-    """All problems through one batched engine from Pi = 1/N; returns (seconds, latencies, X list, stats)."""
+def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing, tag, lanes=1):
+    """All problems from Pi = 1/N through the batched engine; returns (seconds, latencies, X list,
+    results, kernel stats). With lanes > 1 the problems are split by size (M x N, equal counts) into
+    that many batches, each on its own stream and driven by its own host thread, so that small
+    problems advance at their own pace instead of at the pace of the largest ones."""
+    import threading
     for p, c in zip(problems, ctxs):
         c.pset(1.0 / p.N, "X")
-    b = pqdts_amd.Batch(capacity)
-    if timing:
-        b.kernel_timing(True)
-    pending = list(range(len(problems)))
-    slot_of = {}
-    lat, X, res = [None] * len(problems), [None] * len(problems), [None] * len(problems)
+    order = sorted(range(len(problems)), key=lambda i: problems[i].M * problems[i].N)
+    k = max(1, min(lanes, len(problems)))
+    groups = [order[len(order) * g // k: len(order) * (g + 1) // k] for g in range(k)]
+    lat, res = [None] * len(problems), [None] * len(problems)
+    kst, errors = {}, []
+    lock = threading.Lock()
     t0 = time.perf_counter()
-    steps = 0
-    while pending or slot_of:
-        while pending and len(slot_of) < capacity:
-            i = pending.pop(0)
-            slot_of[b.admit(ctxs[i], 2, maxiter, problems[i].eps, 1, 0, precision, record=False)] = i
-        for s in b.step():
-            i = slot_of.pop(s)
-            lat[i] = time.perf_counter() - t0
-            res[i] = b.result(s)
-            b.release(s)
-        steps += 1
-        if steps % 100 == 0:
-            log(f"  {tag}: {steps} steps, {len(pending) + len(slot_of)} unfinished, "
-                f"{time.perf_counter() - t0:.1f} s")
+
+    def lane(g, idx):
+        try:
+            cap = min(capacity, len(idx)) if capacity else len(idx)
+            b = pqdts_amd.Batch(max(cap, 1))
+            if timing:
+                b.kernel_timing(True)
+            pending, slot_of, steps = list(idx), {}, 0
+            while pending or slot_of:
+                while pending and len(slot_of) < cap:
+                    i = pending.pop(0)
+                    slot_of[b.admit(ctxs[i], 2, maxiter, problems[i].eps, 1, 0, precision, record=False)] = i
+                for s in b.step():
+                    i = slot_of.pop(s)
+                    lat[i] = time.perf_counter() - t0
+                    res[i] = b.result(s)
+                    b.release(s)
+                steps += 1
+                if steps % 500 == 0:
+                    log(f"  {tag} lane {g}: {steps} steps, {len(pending) + len(slot_of)} unfinished, "
+                        f"{time.perf_counter() - t0:.1f} s")
+            if timing:
+                with lock:
+                    for name, v in b.kernel_stats().items():
+                        old = kst.get(name, (0, 0.0, 0.0, 0.0))
+                        kst[name] = tuple(x + y for x, y in zip(old, v))
+            b.close()
+        except Exception as e:  # reported by the caller
+            errors.append(e)
+
+    threads = [threading.Thread(target=lane, args=(g, idx)) for g, idx in enumerate(groups) if idx]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
     dt = time.perf_counter() - t0
-    kst = b.kernel_stats() if timing else {}
-    for i, c in enumerate(ctxs):
-        X[i] = c.get("X")
-    b.close()
+    X = [c.get("X") for c in ctxs]
     return dt, lat, X, res, kst
 
 
@@ -242,11 +266,11 @@ def workload_mix(args, pqdts_amd, workloads):
     log(f"uploaded in {time.time() - t:.1f} s; {n_gram} problems take the Gram Hessian product")
     cap = args.capacity or len(problems)
     for w in range(args.warmup):
-        run_batched(pqdts_amd, problems[:8], ctxs[:8], 1, args.maxiter, cap, False, f"warmup {w}")
+        run_batched(pqdts_amd, problems[:8], ctxs[:8], 1, args.maxiter, cap, False, f"warmup {w}", args.lanes)
     mixed_runs = []
     for s in range(args.steps):
         dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, False,
-                                            f"batched mixed step {s}")
+                                            f"batched mixed step {s}", args.lanes)
         nconv = sum(1 for r, p in zip(res, problems) if r["conv"] < p.eps)
         log(f"step {s}: batched mixed {dt:.2f} s, {nconv}/{len(problems)} converged")
         mixed_runs.append((dt, lat, Xm, res, kst, nconv))
@@ -254,14 +278,14 @@ def workload_mix(args, pqdts_amd, workloads):
     dt_m, lat_m, X_m, res_m, _, _ = mixed_runs[0]
     # per-kernel HIP-event timing in a run of its own (not part of `value`)
     dt_k, _, _, _, kst_m = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, True,
-                                       "batched mixed, kernel timing")
+                                       "batched mixed, kernel timing", args.lanes)
     log(f"kernel-timed batched mixed run: {dt_k:.2f} s")
     Lm = []
     for c, x in zip(ctxs, X_m):
         c.put("X", x)
         Lm.append(objective(c))
     dt_b64, lat_b64, X_b64, res_b64, _ = run_batched(pqdts_amd, problems, ctxs, 0, args.maxiter, cap, False,
-                                                     "batched fp64")
+                                                     "batched fp64", args.lanes)
     nconv_b64 = sum(1 for r, p in zip(res_b64, problems) if r["conv"] < p.eps)
     log(f"batched fp64: {dt_b64:.2f} s, {nconv_b64} converged")
     # plain FP64, one problem at a time (pqdts_solve), on every seq_stride-th problem
@@ -289,7 +313,7 @@ def workload_mix(args, pqdts_amd, workloads):
     for c in sctx:
         c.set_hvp_path("F" if c.info()["gram"] else "G")
     _, _, X_alt, _, _ = run_batched(pqdts_amd, sample, sctx, 0, args.maxiter, len(sample), False,
-                                    "fp64, other summation order")
+                                    "fp64, other summation order", args.lanes)
     for c in sctx:
         c.set_hvp_path("auto")
     floor = [float(np.max(np.abs(X64[i] - x))) for i, x in zip(seq_idx, X_alt)]
@@ -302,9 +326,10 @@ def workload_mix(args, pqdts_amd, workloads):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 Hessian products / f64 elsewhere", "data": "synthetic",
         "config": {"workload": f"mix of {len(problems)} uneven problems (M log-uniform [50,2000], N U[M/4,M], "
                                f"D=2M, alpha^2 on [0,0.9M], eta U[0.5,0.95], gamma log-U[1e-5,1e-2], shots "
-                               f"1e6-1e8), eps={args.eps:g}, mode 2, from Pi=1/N, batched engine capacity {cap}, "
-                               f"mixed precision",
-                   "count": len(problems), "eps": args.eps, "maxiter_per_stage": args.maxiter, "capacity": cap},
+                               f"1e6-1e8), eps={args.eps:g}, mode 2, from Pi=1/N, batched engine in {args.lanes} "
+                               f"size lanes (one stream and host thread each), mixed precision",
+                   "count": len(problems), "eps": args.eps, "maxiter_per_stage": args.maxiter, "capacity": cap,
+                   "lanes": args.lanes},
         "converged": {"batched_mixed": nconv_m, "of": len(problems) * args.steps, "batched_fp64": nconv_b64,
                       "plain_fp64_sample": int(sum(conv64))},
         "latency_s": {"p50": pct(lat_m, 50), "p99": pct(lat_m, 99), "max": max(lat_m),
@@ -326,6 +351,9 @@ def workload_mix(args, pqdts_amd, workloads):
             "rel_objective_gap_max": max(gap), "rel_objective_gap_median": pct(gap, 50),
             "note": "fp64 floor = ||dPi||inf between two FP64 solves of the same problem whose Hessian products "
                     "sum in a different order (Gram vs F products): what eps leaves undetermined"},
+        "slowest_mixed": [{"M": problems[i].M, "N": problems[i].N, "gamma": problems[i].gamma,
+                           "newton": res_m[i]["newton"], "pcg": res_m[i]["pcg"], "latency_s": lat_m[i]}
+                          for i in sorted(range(len(problems)), key=lambda k: -lat_m[k])[:5]],
         "solve_stats_mixed": {"newton": int(sum(r["newton"] for r in res_m)),
                               "pcg": int(sum(r["pcg"] for r in res_m)),
                               "max_newton": int(max(r["newton"] for r in res_m))},
@@ -562,6 +590,7 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-7)
     ap.add_argument("--maxiter", type=int, default=5000)
     ap.add_argument("--capacity", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--seq-stride", type=int, default=8)
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--hbm-gb", type=float, default=200.0)
