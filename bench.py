@@ -178,11 +178,13 @@ def cpu_baseline_cfg2(prob, op_counts, threads):
 
 
 # ------------------------------------------------------------------ workloads
-def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing, tag, lanes=1):
+def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing, tag, lanes=1, small_cus=0):
     """All problems from Pi = 1/N through the batched engine; returns (seconds, latencies, X list,
     results, kernel stats). With lanes > 1 the problems are split by size (M x N, equal counts) into
     that many batches, each on its own stream and driven by its own host thread, so that small
-    problems advance at their own pace instead of at the pace of the largest ones."""
+    problems advance at their own pace instead of at the pace of the largest ones. small_cus > 0
+    gives the lane of the smallest problems that many compute units of its own (the others share
+    the rest), so its short, latency-bound launches never queue behind the large lanes' work."""
     import threading
     for p, c in zip(problems, ctxs):
         c.pset(1.0 / p.N, "X")
@@ -198,6 +200,9 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing,
         try:
             cap = min(capacity, len(idx)) if capacity else len(idx)
             b = pqdts_amd.Batch(max(cap, 1))
+            if small_cus and k > 1:
+                ncu = pqdts_amd.device_cu_count()
+                b.set_cus(range(small_cus) if g == 0 else range(small_cus, ncu))
             if timing:
                 b.kernel_timing(True)
             pending, slot_of, steps = list(idx), {}, 0
@@ -266,11 +271,11 @@ def workload_mix(args, pqdts_amd, workloads):
     log(f"uploaded in {time.time() - t:.1f} s; {n_gram} problems take the Gram Hessian product")
     cap = args.capacity or len(problems)
     for w in range(args.warmup):
-        run_batched(pqdts_amd, problems[:8], ctxs[:8], 1, args.maxiter, cap, False, f"warmup {w}", args.lanes)
+        run_batched(pqdts_amd, problems[:8], ctxs[:8], 1, args.maxiter, cap, False, f"warmup {w}", args.lanes, args.small_cus)
     mixed_runs = []
     for s in range(args.steps):
         dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, False,
-                                            f"batched mixed step {s}", args.lanes)
+                                            f"batched mixed step {s}", args.lanes, args.small_cus)
         nconv = sum(1 for r, p in zip(res, problems) if r["conv"] < p.eps)
         log(f"step {s}: batched mixed {dt:.2f} s, {nconv}/{len(problems)} converged")
         mixed_runs.append((dt, lat, Xm, res, kst, nconv))
@@ -278,14 +283,14 @@ def workload_mix(args, pqdts_amd, workloads):
     dt_m, lat_m, X_m, res_m, _, _ = mixed_runs[0]
     # per-kernel HIP-event timing in a run of its own (not part of `value`)
     dt_k, _, _, _, kst_m = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, True,
-                                       "batched mixed, kernel timing", args.lanes)
+                                       "batched mixed, kernel timing", args.lanes, args.small_cus)
     log(f"kernel-timed batched mixed run: {dt_k:.2f} s")
     Lm = []
     for c, x in zip(ctxs, X_m):
         c.put("X", x)
         Lm.append(objective(c))
     dt_b64, lat_b64, X_b64, res_b64, _ = run_batched(pqdts_amd, problems, ctxs, 0, args.maxiter, cap, False,
-                                                     "batched fp64", args.lanes)
+                                                     "batched fp64", args.lanes, args.small_cus)
     nconv_b64 = sum(1 for r, p in zip(res_b64, problems) if r["conv"] < p.eps)
     log(f"batched fp64: {dt_b64:.2f} s, {nconv_b64} converged")
     # plain FP64, one problem at a time (pqdts_solve), on every seq_stride-th problem
@@ -313,7 +318,7 @@ def workload_mix(args, pqdts_amd, workloads):
     for c in sctx:
         c.set_hvp_path("F" if c.info()["gram"] else "G")
     _, _, X_alt, _, _ = run_batched(pqdts_amd, sample, sctx, 0, args.maxiter, len(sample), False,
-                                    "fp64, other summation order", args.lanes)
+                                    "fp64, other summation order", args.lanes, args.small_cus)
     for c in sctx:
         c.set_hvp_path("auto")
     floor = [float(np.max(np.abs(X64[i] - x))) for i, x in zip(seq_idx, X_alt)]
@@ -329,7 +334,7 @@ def workload_mix(args, pqdts_amd, workloads):
                                f"1e6-1e8), eps={args.eps:g}, mode 2, from Pi=1/N, batched engine in {args.lanes} "
                                f"size lanes (one stream and host thread each), mixed precision",
                    "count": len(problems), "eps": args.eps, "maxiter_per_stage": args.maxiter, "capacity": cap,
-                   "lanes": args.lanes},
+                   "lanes": args.lanes, "small_lane_cus": args.small_cus},
         "converged": {"batched_mixed": nconv_m, "of": len(problems) * args.steps, "batched_fp64": nconv_b64,
                       "plain_fp64_sample": int(sum(conv64))},
         "latency_s": {"p50": pct(lat_m, 50), "p99": pct(lat_m, 99), "max": max(lat_m),
@@ -591,6 +596,7 @@ def main():
     ap.add_argument("--maxiter", type=int, default=5000)
     ap.add_argument("--capacity", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=3)
+    ap.add_argument("--small-cus", type=int, default=0)
     ap.add_argument("--seq-stride", type=int, default=8)
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--hbm-gb", type=float, default=200.0)

@@ -67,6 +67,9 @@ const char *pqdts_last_error(void);
 
 /* Number of visible GPUs (0 when none); never fails. */
 int pqdts_device_count(void);
+/* Compute units of the current device (256 on MI355X). No counterpart in the
+ * reference. */
+int pqdts_device_cu_count(void);
 
 /* Builds a problem on the current GPU from COO F (D x M) and P (D x N):
  * the reference's F ingest (pqdts.f90:1318-1365), P densify (1527-1538) and
@@ -123,7 +126,8 @@ int pqdts_pcopy_scale(pqdts_ctx *ctx, double alpha, int x, int y);
 /* reduced-space transform of X (pqdts.f90:1970-1980) */
 int pqdts_transform(pqdts_ctx *ctx);
 /* Line searches along slot SN from slot X, trial points in XTMP:
- * Lalpha_min2 (pqdts.f90:394-438) and Lalpha_min_auglag (pqdts.f90:440-483). */
+ * Lalpha_min2 (pqdts.f90:394-438) and Lalpha_min_auglag (pqdts.f90:440-483).
+ * On return XTMP holds the last trial point evaluated. */
 int pqdts_lalpha_min2(pqdts_ctx *ctx, double *alpha, double *dlalpha);
 int pqdts_lalpha_min_auglag(pqdts_ctx *ctx, double *alpha, double *dlalpha);
 
@@ -198,6 +202,11 @@ int pqdts_batch_step(pqdts_batch *b, int *finished, int cap);
 int pqdts_batch_result(pqdts_batch *b, int slot, double *conv, int *stats);
 int pqdts_batch_release(pqdts_batch *b, int slot);
 int pqdts_batch_active(pqdts_batch *b);
+/* Restricts an empty batch's launches to a subset of the device's compute units:
+ * bit i of mask[i / 32] selects CU i (nwords 32-bit words). Lets a server keep
+ * small, latency-bound problems in their own batch on CUs that large problems'
+ * batches do not use. No counterpart in the reference. */
+int pqdts_batch_set_cu_mask(pqdts_batch *b, const uint32_t *mask, int nwords);
 int pqdts_batch_kernel_timing(pqdts_batch *b, int on);
 int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms,
                              double *flops, double *bytes);

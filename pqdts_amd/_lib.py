@@ -29,6 +29,7 @@ EXPORTS = [
     "pqdts_kernel_timing", "pqdts_kernel_stats", "pqdts_info", "pqdts_set_hvp_path",
     "pqdts_batch_create", "pqdts_batch_destroy", "pqdts_batch_admit", "pqdts_batch_step", "pqdts_batch_result",
     "pqdts_batch_release", "pqdts_batch_active", "pqdts_batch_kernel_timing", "pqdts_batch_kernel_stats",
+    "pqdts_device_cu_count", "pqdts_batch_set_cu_mask",
 ]
 
 _lib = None
@@ -49,6 +50,8 @@ def load():
     i32p = C.POINTER(C.c_int32)
     L.pqdts_last_error.restype = C.c_char_p
     L.pqdts_device_count.restype = i32
+    L.pqdts_device_cu_count.restype = i32
+    L.pqdts_batch_set_cu_mask.argtypes = [vp, C.POINTER(C.c_uint32), i32]
     L.pqdts_create.argtypes = [i64, i64, i64, i64, i32p, i32p, dp, i64, i32p, i32p, dp, d, C.POINTER(vp)]
     L.pqdts_destroy.argtypes = [vp]
     L.pqdts_put.argtypes = [vp, i32, dp, i32]
@@ -109,6 +112,10 @@ def _p(a, t=C.c_double):
 
 def device_count() -> int:
     return int(load().pqdts_device_count())
+
+
+def device_cu_count() -> int:
+    return int(load().pqdts_device_cu_count())
 
 
 def simplexproj_rows(y: np.ndarray) -> np.ndarray:
@@ -386,6 +393,14 @@ class Batch:
 
     def active(self) -> int:
         return int(load().pqdts_batch_active(self.h))
+
+    def set_cus(self, cus):
+        """Restricts this (empty) batch to the compute units listed in `cus`."""
+        n = max(cus) // 32 + 1
+        words = (C.c_uint32 * n)()
+        for c in cus:
+            words[c // 32] |= 1 << (c % 32)
+        _check(load().pqdts_batch_set_cu_mask(self.h, words, n))
 
     def kernel_timing(self, on: bool):
         _check(load().pqdts_batch_kernel_timing(self.h, int(bool(on))))

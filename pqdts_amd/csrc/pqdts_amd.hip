@@ -65,8 +65,12 @@ constexpr int KC = 16;    // band chunk staged per LDS buffer (4 MFMA k-steps)
 // device scalar slots per problem
 enum {
   S_LA = 0, S_LB, S_XMIN, S_NACT, S_ND, S_B1, S_V2, S_RZ0, S_RZ1, S_RR, S_CONV, S_DOT,
-  S_EPSCG, S_CGACT, S_CGN  // device-side CG loop: eps, still-iterating flag, rounds done
+  S_EPSCG, S_CGACT, S_CGN,  // device-side CG loop: eps, still-iterating flag, rounds done
+  S_LT = 16,                 // line-search trial k: data term S_LT + k, smoothing S_LTB + k, xmin S_LTX + k
+  S_LTB = S_LT + LS_K,
+  S_LTX = S_LTB + LS_K
 };
+static_assert(S_CGN < S_LT && S_LTX + LS_K <= NSCAL, "scalar slots");
 // partial-sum regions per problem
 enum { R_A = 0, R_B = 1, R_C = 2 };
 
@@ -246,6 +250,15 @@ struct Engine {
 
   ~Engine();
 
+  // Replaces the engine's own stream by one restricted to the CUs of `mask`.
+  void set_cu_mask(const uint32_t *mask, int nwords) {
+    require(own_stream, "only an engine with its own stream can be restricted");
+    HIPC(hipStreamSynchronize(s));
+    hipStream_t ns = nullptr;
+    HIPC(hipExtStreamCreateWithCUMask(&ns, (uint32_t)nwords, mask));
+    (void)hipStreamDestroy(s);
+    s = ns;
+  }
   void init(int capacity, hipStream_t stream, bool make_stream) {
     cap = capacity;
     ctx.assign(cap, nullptr);
@@ -483,8 +496,8 @@ struct Engine {
            [&](const WorkList &wl, int tot) { kb_reconstruct<<<tot, BLOCK, 0, s>>>(dprobs, wl, x); });
   }
 
-  // L (pqdts.f90:917-973): scalars S_LA (data term) and S_LB (smoothing)
-  void L(const std::vector<int> &S, int x, bool recon) {
+  // L (pqdts.f90:917-973): data term -> scal[ia], smoothing -> scal[ib]
+  void L(const std::vector<int> &S, int x, bool recon, int ia = S_LA, int ib = S_LB) {
     struct Scope sc(this, "L");
     if (recon) reconstruct(S, x);
     GemmOps ops{BAND_F, x, OP_O, 0, 0, R_A};
@@ -497,7 +510,7 @@ struct Engine {
     launch(S, [&](int p) { return std::max(1, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); },
            [&](const WorkList &wl, int tot) { kb_smooth<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, R_B); });
     launch(S, [](int) { return 1; }, [&](const WorkList &wl, int) {
-      kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 2, R_A, S_LA, R_B, S_LB, 0, 0);
+      kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 2, R_A, ia, R_B, ib, 0, 0);
     });
   }
   double Lval(int p) { return host_scal(p)[S_LA] + host_scal(p)[S_LB]; }
@@ -590,7 +603,7 @@ struct Engine {
 
   // doproj / doproj2 (pqdts.f90:1120-1164) on operand t; with form_trial the
   // trial point t = arg*SN + X is formed first. xmin -> S_XMIN.
-  void project(const std::vector<int> &S, int t, bool form_trial, bool recon_again) {
+  void project(const std::vector<int> &S, int t, bool form_trial, bool recon_again, int ixmin = S_XMIN) {
     need_maxr(S);
     bool any_full = false, any_red = false;
     for (int p : S) (ctx[p]->full ? any_full : any_red) = true;
@@ -598,7 +611,7 @@ struct Engine {
     hipEvent_t e0 = kbegin();
     launch_reduce(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
       kb_project<<<tot, BLOCK, 0, s>>>(dprobs, wl, t, form_trial ? 1 : 0, recon_again ? 1 : 0, R_C);
-    }, R_C, S_XMIN, true);
+    }, R_C, ixmin, true);
     kend(e0, "project", S, [&](int p) { return stream_work(p, form_trial ? 24.0 : 16.0); });
   }
 
@@ -726,48 +739,69 @@ bool Engine::any_active() const {
 // Lalpha_min_auglag (stage 1, pqdts.f90:440-483) for full-space problems and
 // Lalpha_min2 (pqdts.f90:394-438) for reduced ones, in lockstep: every round
 // evaluates one trial point for each problem still searching.
+// Lalpha_min2 / Lalpha_min_auglag (pqdts.f90:394-483) for every problem of A. The
+// trials of a round run back to back on the device, each into its own scalar
+// slots, and are read with one host round trip; the host walks them in the
+// reference's order and stops at the first accepted one, so trial values, counts
+// and events do not depend on the round size. The two evaluations at alpha = 0
+// and dx share a round; the Armijo trials go one per round (LS_ROUND): evaluating
+// more ahead cost the batched mix more GPU time than the round trips it saved.
 void Engine::line_search(const std::vector<int> &A) {
   Scope sc(this, "Lalpha_min");
   const double dx = (double)1e-5f;      // pqdts.f90:354/404/450
   const double feas = (double)-1e-12f;  // pqdts.f90:424/430
-  auto trial = [&](const std::vector<int> &S) {
-    project(S, PQDTS_XTMP, true, true);
-    L(S, PQDTS_XTMP, false);  // the reduced-space prep was applied by kb_project
+  // trials k = 0..K-1 of every problem of S at step lengths alpha(p, k)
+  auto trials = [&](const std::vector<int> &S, int K, const std::function<double(int, int)> &alpha) {
+    for (int k = 0; k < K; k++) {
+      for (int p : S) arg[p] = alpha(p, k);
+      project(S, PQDTS_XTMP, true, true, S_LTX + k);
+      L(S, PQDTS_XTMP, false, S_LT + k, S_LTB + k);  // the reduced-space prep was applied by kb_project
+    }
     fetch();
   };
-  for (int p : A) arg[p] = 0.0;
-  trial(A);
-  for (int p : A) st[p]->ls_L0 = Lval(p);
-  for (int p : A) arg[p] = 0.0 + dx;
-  trial(A);
+  auto Lk = [&](int p, int k) { return host_scal(p)[S_LT + k] + host_scal(p)[S_LTB + k]; };
+  // L at alpha = 0 and at alpha = dx (the slope probe) as one round
+  trials(A, 2, [&](int, int k) { return k == 0 ? 0.0 : 0.0 + dx; });
   for (int p : A) {
     PState &q = *st[p];
-    q.ls_df = (Lval(p) - q.ls_L0) / dx;
+    q.ls_L0 = Lk(p, 0);
+    q.ls_df = (Lk(p, 1) - q.ls_L0) / dx;
     q.dlalpha = q.ls_df;
     q.ls_alpha = 1.0;
     q.ls_lm = std::numeric_limits<double>::max();
     q.ls_amin = 0.0;
   }
   std::vector<int> S = A;
-  for (int i = 1; i <= 100 && !S.empty(); i++) {
-    for (int p : S) arg[p] = st[p]->ls_alpha;
-    trial(S);
+  constexpr int LS_ROUND = 1;
+  for (int i = 1; i <= 100 && !S.empty(); i += LS_ROUND) {
+    const int Kr = std::min(LS_ROUND, 101 - i);
+    // alpha of trial k: the current one times 0.75^k, multiplied step by step as the
+    // reference does
+    trials(S, Kr, [&](int p, int k) {
+      double a = st[p]->ls_alpha;
+      for (int t = 0; t < k; t++) a = a * 0.75;
+      return a;
+    });
     std::vector<int> next;
     for (int p : S) {
       PState &q = *st[p];
       const bool min2 = !ctx[p]->full;
-      const double Lp = Lval(p), xmin = min2 ? host_scal(p)[S_XMIN] : 0.0;
-      const bool feasible = !min2 || xmin >= feas;
-      q.trials++;
-      if (Lp < q.ls_lm && feasible) {
-        q.ls_lm = Lp;
-        q.ls_amin = q.ls_alpha;
+      bool accepted = false;
+      for (int k = 0; k < Kr && !accepted; k++) {
+        const double Lp = Lk(p, k), xmin = min2 ? host_scal(p)[S_LTX + k] : 0.0;
+        const bool feasible = !min2 || xmin >= feas;
+        q.trials++;
+        if (Lp < q.ls_lm && feasible) {
+          q.ls_lm = Lp;
+          q.ls_amin = q.ls_alpha;
+        }
+        q.emit(PQDTS_EV_LS, {(double)(i + k), Lp, q.ls_alpha, q.ls_df, xmin, min2 ? 1.0 : 0.0});
+        if (Lp > q.ls_L0 + 0.1 * q.ls_alpha * q.ls_df || !feasible)
+          q.ls_alpha = q.ls_alpha * 0.75;
+        else
+          accepted = true;
       }
-      q.emit(PQDTS_EV_LS, {(double)i, Lp, q.ls_alpha, q.ls_df, xmin, min2 ? 1.0 : 0.0});
-      if (Lp > q.ls_L0 + 0.1 * q.ls_alpha * q.ls_df || !feasible) {
-        q.ls_alpha = q.ls_alpha * 0.75;
-        next.push_back(p);
-      }
+      if (!accepted) next.push_back(p);
     }
     S.swap(next);
   }
@@ -1323,6 +1357,13 @@ int pqdts_device_count(void) {
   return n;
 }
 
+int pqdts_device_cu_count(void) {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return n;
+}
+
 int pqdts_create(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *F_row, const int32_t *F_col,
                  const double *F_val, int64_t nP, const int32_t *P_row, const int32_t *P_col, const double *P_val,
                  double gamma, pqdts_ctx **out) {
@@ -1802,6 +1843,17 @@ int pqdts_batch_release(pqdts_batch *b, int slot) {
     require(b != nullptr && slot >= 0 && slot < b->eng.cap, "bad slot");
     HIPC(hipStreamSynchronize(b->eng.s));
     b->eng.detach(slot);
+  });
+}
+
+int pqdts_batch_set_cu_mask(pqdts_batch *b, const uint32_t *mask, int nwords) {
+  return guarded([&] {
+    require(b != nullptr && mask != nullptr && nwords >= 1, "bad CU mask arguments");
+    bool any = false;
+    for (int i = 0; i < nwords; i++) any = any || mask[i] != 0u;
+    require(any, "the CU mask selects no CU");
+    for (int p = 0; p < b->eng.cap; p++) require(b->eng.ctx[p] == nullptr, "the batch must be empty");
+    b->eng.set_cu_mask(mask, nwords);
   });
 }
 

@@ -105,3 +105,19 @@ def test_mixed_precision_within_1e6_of_fp64(path):
     assert r64["conv"] < eps and r32["conv"] < eps
     assert dpi <= 1e-6
     assert gap <= 1e-9
+
+
+def test_cu_restricted_batch_matches_alone():
+    """A batch limited to a subset of the compute units gives the same bits."""
+    p = problems()[1]
+    Xa, conva, _ = solve_alone(p)
+    c = pqdts_amd.Context.from_problem(p)
+    start(c, p)
+    b = pqdts_amd.Batch(2)
+    b.set_cus(range(8, 40))
+    b.admit(c, 2, 300, p.eps, 1, 0, 0)
+    r = b.run()[0]
+    assert np.array_equal(c.get("X"), Xa)
+    assert r["conv"] == conva
+    with pytest.raises(pqdts_amd.PqdtsError):
+        b.set_cus(range(4))  # only an empty batch can be restricted
