@@ -379,6 +379,10 @@ program pqdts_amd_driver
     stop
   end if
 
+  ! the generator is seeded once, at program start (pqdts.f90:1241); P (when random)
+  ! and the rows of the maxr X then draw from it in the reference's order
+  call random_seed_fixed()
+
   ! ---- F: from data/F_*.bin (pqdts.f90:1318-1365) or generated (1366-1466)
   if (NF >= 0) then
     call read_coo("data/F", NF, Fr, Fc, Fv)
@@ -397,7 +401,7 @@ program pqdts_amd_driver
   else
     nnzP = N*D
     allocate (Pr(nnzP), Pc(nnzP), Pv(nnzP))
-    call random_seed_fixed()
+    ! P(N, D) of the reference in memory order: n fastest, then d
     call random_number(Pv)
     k = 0
     do j = 0, D - 1
@@ -413,12 +417,11 @@ program pqdts_amd_driver
   deallocate (Fr, Fc, Fv, Pr, Pc, Pv)
   allocate (X(M, N))
 
-  ! maxr from a random row-stochastic X (pqdts.f90:1565-1575); it only matters
-  ! when the solve starts in stage 2
+  ! maxr from a random row-stochastic X, one row at a time as the reference draws it
+  ! (pqdts.f90:1565-1575); it only matters when the solve starts in stage 2
   allocate (maxr0(M))
-  call random_seed_fixed()
-  call random_number(X)
   do i = 1, M
+    call random_number(X(i, :))
     X(i, :) = X(i, :)/sum(X(i, :))
     maxr0(i) = int(maxloc(X(i, :), 1) - 1, int32)
   end do

@@ -12,6 +12,8 @@ Fixtures
   cfg1_ref_states.npz     X after stage 1 (eps 1e-6, 1 thread) and the final X at eps 1e-11 with
                           1 and 4 threads, with their objective values
   condat_vectors.npz      inputs and outputs of the reference's condat_simplexproj.c
+  driver_ref.json         reference stdout events of the drop-in cases of driver_cases.py
+  driver_ref_states.npz   and the final X they wrote (rank_     0_final.dat)
 """
 import hashlib
 import json
@@ -27,6 +29,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
 
+import driver_cases  # noqa: E402
 import oracle  # noqa: E402
 import reftrace  # noqa: E402
 from pqdts_amd import fortio, workloads  # noqa: E402
@@ -102,6 +105,20 @@ def main():
         xs.append(oracle.ref_condat(y))
     np.savez_compressed(os.path.join(HERE, "condat_vectors.npz"),
                         **{f"y{i}": y for i, y in enumerate(ys)}, **{f"x{i}": x for i, x in enumerate(xs)})
+
+    events, finals = {}, {}
+    for name in driver_cases.CASES:
+        with tempfile.TemporaryDirectory() as w:
+            args = driver_cases.build(name, w)
+            env = dict(os.environ, OMP_NUM_THREADS="1", FORT_FMT_RECL="100000")
+            r = subprocess.run([oracle.REF_BIN] + [str(a) for a in args], cwd=w, env=env, stdout=subprocess.PIPE,
+                               text=True, check=True)
+            events[name] = {"args": [str(a) for a in args],
+                            "events": [[k, list(v)] for k, v in reftrace.parse(r.stdout)]}
+            _, _, _, _, _, finals[name] = fortio.read_state(os.path.join(w, "rank_     0_final.dat"))
+    with open(os.path.join(HERE, "driver_ref.json"), "w") as f:
+        json.dump(events, f)
+    np.savez_compressed(os.path.join(HERE, "driver_ref_states.npz"), **finals)
     print("wrote fixtures to", HERE)
 
 

@@ -125,3 +125,55 @@ def test_driver_internal_F(tmp_path):
     assert "Fsize" in out
     _, _, _, _, _, X = pqdts_py_final(str(tmp_path))
     assert np.allclose(X.sum(1), 1.0, atol=1e-12)
+
+
+# ---- cases pinned by the reference binary (tests/golden/driver_cases.py) -------------
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _gold():
+    import json
+    with open(os.path.join(GOLD, "driver_ref.json")) as f:
+        ev = json.load(f)
+    return ev, np.load(os.path.join(GOLD, "driver_ref_states.npz"))
+
+
+def _close(a, b, rel):
+    return abs(a - b) <= rel * max(abs(a), abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["intF", "smooth", "mode1", "randomP", "stage2"])
+def test_driver_matches_reference_case(tmp_path, name):
+    """The new binary against the reference binary's own run on the same inputs:
+    the same events (PCG and line-search counts, step lengths, active sets) and,
+    within the reference's reproducible prefix, the same objective values and Pi."""
+    import sys
+    sys.path.insert(0, GOLD)
+    import driver_cases
+    import reftrace
+    ev_ref, states = _gold()
+    args = driver_cases.build(name, str(tmp_path))
+    assert [str(a) for a in args] == ev_ref[name]["args"]
+    cmd = [BIN] + [str(a) for a in args]
+    r = subprocess.run(cmd, cwd=str(tmp_path), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    got = reftrace.parse(r.stdout)
+    ref = [(k, tuple(v)) for k, v in ev_ref[name]["events"]]
+    assert [k for k, _ in got] == [k for k, _ in ref], name
+    for (k, v), (_, w) in zip(got, ref):
+        if k == reftrace.EV_ITER:
+            assert v[0] == w[0] and v[1] == w[1] and v[6] == w[6]
+            assert _close(v[2], w[2], 1e-10), (name, v, w)        # L
+            assert _close(v[7], w[7], 1e-12), (name, v, w)        # alpha
+            assert _close(v[8], w[8], 1e-6), (name, v, w)         # dlalpha (finite difference)
+        elif k == reftrace.EV_ITER0:
+            assert _close(v[0], w[0], 1e-12), (name, v, w)
+        elif k == reftrace.EV_LS:
+            assert v[0] == w[0] and _close(v[1], w[1], 1e-10) and _close(v[2], w[2], 1e-12), (name, v, w)
+        elif k == reftrace.EV_ACTIVE:
+            assert v == w, (name, v, w)
+        elif k == reftrace.EV_PCG:
+            assert v[0] == w[0] and _close(v[1], w[1], 1e-6), (name, v, w)
+    _, _, _, _, _, X = fortio.read_state(str(tmp_path / "rank_     0_final.dat"))
+    assert np.max(np.abs(X - states[name])) <= 1e-9, (name, np.max(np.abs(X - states[name])))

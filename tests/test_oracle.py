@@ -319,3 +319,23 @@ def test_oracle_tight_endpoint(cfg1):
     assert np.max(np.abs(X - X1)) <= max(1e-6, 20 * spread)
     its = [v for k, v in ev if k == reftrace.EV_ITER]
     assert _rel(its[-1][2], float(z["final_eps1e-11_t1_L"])) <= 1e-9
+
+
+def test_oracle_mode1_matches_reference(cfg1):
+    """Mode 1 (projected gradient, pqdts.f90:1934-1952), three iterations per
+    stage, event by event against the reference binary's run of the same case
+    (tests/golden/driver_ref.json, case mode1: cfg 1 from Pi = 1/N)."""
+    ref = json.load(open(os.path.join(GOLD, "driver_ref.json")))["mode1"]["events"]
+    o = oracle.Oracle.from_problem(cfg1)
+    o.buf("X")[:] = 1.0 / cfg1.N
+    _, ev = o.solve(1, 3, 1e-6, 1, 0)
+    ev = [(k, v) for k, v in ev if k in (reftrace.EV_ITER0, reftrace.EV_LS, reftrace.EV_ITER, reftrace.EV_STOP1)]
+    ref = [(k, v) for k, v in ref if k in (reftrace.EV_ITER0, reftrace.EV_LS, reftrace.EV_ITER, reftrace.EV_STOP1)]
+    assert [k for k, _ in ev] == [k for k, _ in ref]
+    for (k, a), (_, b) in zip(ev, ref):
+        a, b = np.array(a), np.array(b)
+        if k == reftrace.EV_LS:
+            assert a[0] == b[0] and a[2] == b[2] and _rel(a[1], b[1]) <= 1e-12
+        elif k == reftrace.EV_ITER:
+            assert tuple(a[:2]) == tuple(b[:2]) and a[7] == b[7]
+            assert _rel(a[2], b[2]) <= 1e-12 and _rel(a[4], b[4]) <= 1e-9
