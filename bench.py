@@ -178,13 +178,14 @@ def cpu_baseline_cfg2(prob, op_counts, threads):
 
 
 # ------------------------------------------------------------------ workloads
-def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing, tag, lanes=1, small_cus=0):
+def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing, tag, lanes=1, lane_cus=None):
     """All problems from Pi = 1/N through the batched engine; returns (seconds, latencies, X list,
     results, kernel stats). With lanes > 1 the problems are split by size (M x N, equal counts) into
     that many batches, each on its own stream and driven by its own host thread, so that small
-    problems advance at their own pace instead of at the pace of the largest ones. small_cus > 0
-    gives the lane of the smallest problems that many compute units of its own (the others share
-    the rest), so its short, latency-bound launches never queue behind the large lanes' work."""
+    problems advance at their own pace instead of at the pace of the largest ones. lane_cus (one
+    count per lane, smallest problems first) gives every lane compute units of its own: the
+    small lanes' short, latency-bound launches then never wait for CUs held by the long
+    workgroups of the large problems' products."""
     import threading
     for p, c in zip(problems, ctxs):
         c.pset(1.0 / p.N, "X")
@@ -200,9 +201,9 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing,
         try:
             cap = min(capacity, len(idx)) if capacity else len(idx)
             b = pqdts_amd.Batch(max(cap, 1))
-            if small_cus and k > 1:
-                ncu = pqdts_amd.device_cu_count()
-                b.set_cus(range(small_cus) if g == 0 else range(small_cus, ncu))
+            if lane_cus and k > 1:
+                first = sum(lane_cus[:g])
+                b.set_cus(range(first, first + lane_cus[g]))
             if timing:
                 b.kernel_timing(True)
             pending, slot_of, steps = list(idx), {}, 0
@@ -258,6 +259,15 @@ def calibration_copy(pqdts_amd, problems, ctxs):
     return nbytes
 
 
+def parse_lane_cus(args, pqdts_amd):
+    if not args.lane_cus:
+        return None
+    cus = [int(x) for x in args.lane_cus.split(",")]
+    if len(cus) != args.lanes or sum(cus) > pqdts_amd.device_cu_count():
+        raise SystemExit(f"--lane-cus needs {args.lanes} counts summing to at most the device's CUs")
+    return cus
+
+
 def workload_mix(args, pqdts_amd, workloads):
     t = time.time()
     problems = workloads.batch_mix(args.count, eps=args.eps)
@@ -270,12 +280,13 @@ def workload_mix(args, pqdts_amd, workloads):
     n_gram = sum(1 for c in ctxs if c.info()["gram"])
     log(f"uploaded in {time.time() - t:.1f} s; {n_gram} problems take the Gram Hessian product")
     cap = args.capacity or len(problems)
+    lane_cus = parse_lane_cus(args, pqdts_amd)
     for w in range(args.warmup):
-        run_batched(pqdts_amd, problems[:8], ctxs[:8], 1, args.maxiter, cap, False, f"warmup {w}", args.lanes, args.small_cus)
+        run_batched(pqdts_amd, problems[:8], ctxs[:8], 1, args.maxiter, cap, False, f"warmup {w}", args.lanes, lane_cus)
     mixed_runs = []
     for s in range(args.steps):
         dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, cap, False,
-                                            f"batched mixed step {s}", args.lanes, args.small_cus)
+                                            f"batched mixed step {s}", args.lanes, lane_cus)
         nconv = sum(1 for r, p in zip(res, problems) if r["conv"] < p.eps)
         log(f"step {s}: batched mixed {dt:.2f} s, {nconv}/{len(problems)} converged")
         mixed_runs.append((dt, lat, Xm, res, kst, nconv))
@@ -292,7 +303,7 @@ def workload_mix(args, pqdts_amd, workloads):
         c.put("X", x)
         Lm.append(objective(c))
     dt_b64, lat_b64, X_b64, res_b64, _ = run_batched(pqdts_amd, problems, ctxs, 0, args.maxiter, cap, False,
-                                                     "batched fp64", args.lanes, args.small_cus)
+                                                     "batched fp64", args.lanes, lane_cus)
     nconv_b64 = sum(1 for r, p in zip(res_b64, problems) if r["conv"] < p.eps)
     log(f"batched fp64: {dt_b64:.2f} s, {nconv_b64} converged")
     # plain FP64, one problem at a time (pqdts_solve), on every seq_stride-th problem
@@ -320,7 +331,7 @@ def workload_mix(args, pqdts_amd, workloads):
     for c in sctx:
         c.set_hvp_path("F" if c.info()["gram"] else "G")
     _, _, X_alt, _, _ = run_batched(pqdts_amd, sample, sctx, 0, args.maxiter, len(sample), False,
-                                    "fp64, other summation order", args.lanes, args.small_cus)
+                                    "fp64, other summation order", args.lanes, lane_cus)
     for c in sctx:
         c.set_hvp_path("auto")
     floor = [float(np.max(np.abs(X64[i] - x))) for i, x in zip(seq_idx, X_alt)]
@@ -336,7 +347,7 @@ def workload_mix(args, pqdts_amd, workloads):
                                f"1e6-1e8), eps={args.eps:g}, mode 2, from Pi=1/N, batched engine in {args.lanes} "
                                f"size lanes (one stream and host thread each), mixed precision",
                    "count": len(problems), "eps": args.eps, "maxiter_per_stage": args.maxiter, "capacity": cap,
-                   "lanes": args.lanes, "small_lane_cus": args.small_cus},
+                   "lanes": args.lanes, "lane_cus": lane_cus},
         "converged": {"batched_mixed": nconv_m, "of": len(problems) * args.steps, "batched_fp64": nconv_b64,
                       "plain_fp64_sample": int(sum(conv64))},
         "latency_s": {"p50": pct(lat_m, 50), "p99": pct(lat_m, 99), "max": max(lat_m),
@@ -641,12 +652,13 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["mix", "cfg2", "cfg3", "served"], default="mix")
-    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--count", type=int, default=0, help="problems (mix: 256) or requests (served: 64)")
     ap.add_argument("--eps", type=float, default=1e-7)
     ap.add_argument("--maxiter", type=int, default=5000)
     ap.add_argument("--capacity", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=3)
-    ap.add_argument("--small-cus", type=int, default=0)
+    ap.add_argument("--lane-cus", default="",
+                    help="compute units per lane, smallest problems first (e.g. 48,80,128); empty: shared")
     ap.add_argument("--seq-stride", type=int, default=8)
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--hbm-gb", type=float, default=200.0)
@@ -674,8 +686,10 @@ def main():
             args.maxiter = 3
         out = workload_cfg3(args, pqdts_amd, workloads)
     elif args.workload == "served":
+        args.count = args.count or 64
         out = workload_served(args, pqdts_amd, workloads)
     else:
+        args.count = args.count or 256
         out = workload_mix(args, pqdts_amd, workloads)
     print(json.dumps(out), flush=True)
 
