@@ -178,6 +178,16 @@ def cpu_baseline_cfg2(prob, op_counts, threads):
 
 
 # ------------------------------------------------------------------ workloads
+HVP_SPLIT = -1  # --hvp-split: -1 leaves the library's choice
+
+
+def make_ctx(pqdts_amd, prob):
+    c = pqdts_amd.Context.from_problem(prob)
+    if HVP_SPLIT >= 0:
+        c.set_hvp_split(HVP_SPLIT)
+    return c
+
+
 def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing, tag, lanes=1, lane_cus=None):
     """All problems from Pi = 1/N through the batched engine; returns (seconds, latencies, X list,
     results, kernel stats). With lanes > 1 the problems are split by size (M x N, equal counts) into
@@ -220,6 +230,7 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing,
                 if steps % 500 == 0:
                     log(f"  {tag} lane {g}: {steps} steps, {len(pending) + len(slot_of)} unfinished, "
                         f"{time.perf_counter() - t0:.1f} s")
+            log(f"  {tag} lane {g}: {len(idx)} problems done in {steps} steps, {time.perf_counter() - t0:.1f} s")
             if timing:
                 with lock:
                     for name, v in b.kernel_stats().items():
@@ -276,7 +287,7 @@ def workload_mix(args, pqdts_amd, workloads):
     if pqdts_amd.device_count() < 1:
         raise SystemExit("no GPU visible")
     t = time.time()
-    ctxs = [pqdts_amd.Context.from_problem(p) for p in problems]
+    ctxs = [make_ctx(pqdts_amd, p) for p in problems]
     n_gram = sum(1 for c in ctxs if c.info()["gram"])
     log(f"uploaded in {time.time() - t:.1f} s; {n_gram} problems take the Gram Hessian product")
     cap = args.capacity or len(problems)
@@ -398,7 +409,7 @@ def workload_cfg2(args, pqdts_amd, workloads):
     log(f"generated cfg2: M={prob.M} N={prob.N} D={prob.D} nnz={prob.nnz} in {time.time() - t:.1f} s")
     if pqdts_amd.device_count() < 1:
         raise SystemExit("no GPU visible")
-    ctx = pqdts_amd.Context.from_problem(prob)
+    ctx = make_ctx(pqdts_amd, prob)
     info = ctx.info()
 
     def progress(_c, kind, v):
@@ -436,7 +447,7 @@ def workload_cfg2(args, pqdts_amd, workloads):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "cfg2 plain FP64 solve, one problem at a time", "M": prob.M, "N": prob.N,
                    "D": prob.D, "nnzF": prob.nnz, "gamma": prob.gamma, "eps": prob.eps, "mode": 2,
-                   "hvp": "gram" if info["gram"] else "F products"},
+                   "hvp": "gram" if info["gram"] else "F products", "hvp_split": info["hvp_split"]},
         "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat)},
         "solve": {"newton_iterations": len(its), "pcg_iterations": npcg, "line_search_trials": nls,
                   "final_L": its[-1][2], "final_conv": conv, "converged": bool(conv < prob.eps), "op_counts": counts},
@@ -460,7 +471,7 @@ def workload_cfg3(args, pqdts_amd, workloads):
     if pqdts_amd.device_count() < 1:
         raise SystemExit("no GPU visible")
     t = time.time()
-    ctx = pqdts_amd.Context.from_problem(prob)
+    ctx = make_ctx(pqdts_amd, prob)
     t_up = time.time() - t
     info = ctx.info()
     log(f"context in {t_up:.1f} s: {info}")
@@ -564,7 +575,7 @@ def workload_served(args, pqdts_amd, workloads):
                                 break
                             used[0] += footprint(reqs[i])
                         queue.pop(0)
-                        c = pqdts_amd.Context.from_problem(reqs[i])
+                        c = make_ctx(pqdts_amd, reqs[i])
                         c.pset(1.0 / reqs[i].N, "X")
                         live[b.admit(c, 2, args.maxiter, reqs[i].eps, 1, 0, precision, record=False)] = (i, c)
                     if not live:
@@ -599,7 +610,7 @@ def workload_served(args, pqdts_amd, workloads):
             now = time.perf_counter() - t0
             if arrivals[i] > now:
                 time.sleep(arrivals[i] - now)
-            c = pqdts_amd.Context.from_problem(reqs[i])
+            c = make_ctx(pqdts_amd, reqs[i])
             c.pset(1.0 / reqs[i].N, "X")
             c.solve(2, args.maxiter, reqs[i].eps, 1, 0)
             lat[i] = time.perf_counter() - t0 - arrivals[i]
@@ -631,7 +642,7 @@ def workload_served(args, pqdts_amd, workloads):
 def kernels_probe(args, pqdts_amd, workloads):
     """Each hot operator a few times on cfg2 plus a 16 MB copy, for profiler passes."""
     prob = workloads.config2()
-    ctx = pqdts_amd.Context.from_problem(prob)
+    ctx = make_ctx(pqdts_amd, prob)
     rng = np.random.default_rng(0)
     X = rng.random((prob.M, prob.N))
     ctx.put("X", X / X.sum(1, keepdims=True))
@@ -667,7 +678,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--kernels", type=int, default=0)
     ap.add_argument("--calib-copy", action="store_true")
+    ap.add_argument("--hvp-split", type=int, default=-1,
+                    help="split of the Gram product over band chunks (0 by shape, 1 none, k fixed)")
     args = ap.parse_args()
+    global HVP_SPLIT
+    HVP_SPLIT = args.hvp_split
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems shard "
               f"one per GPU with no communication, see DESIGN.md", file=sys.stderr)

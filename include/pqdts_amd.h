@@ -162,11 +162,12 @@ int pqdts_kernel_timing(pqdts_ctx *ctx, int on);
 int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms, double *flops,
                        double *bytes);
 
-/* Sizes of a context: info[0..11] = M, N, D, stored F elements (sum of row span
+/* Sizes of a context: info[0..12] = M, N, D, stored F elements (sum of row span
  * lengths, the reference's Frows storage), packed elements of F's band tiles,
  * #F tiles, #F^T tiles, Gram path active (0/1), packed elements of G's band
  * tiles, stored elements of G's row bands, packed elements of F^T's band
- * tiles, device bytes held by the problem. */
+ * tiles, device bytes held by the problem, split of the Gram product over band
+ * chunks (1 = none; see pqdts_set_hvp_split). */
 int pqdts_info(pqdts_ctx *ctx, int64_t *info);
 
 /* How BF (and every CG step) applies the Hessian: 1 = the reference's two
@@ -175,6 +176,17 @@ int pqdts_info(pqdts_ctx *ctx, int64_t *info);
  * cost less than 0.8 of the two products). Same operator, different rounding.
  * No counterpart in the reference. */
 int pqdts_set_hvp_path(pqdts_ctx *ctx, int path);
+
+/* Split of the Gram product over the band chunks of each output tile: split k
+ * has k workgroups multiply disjoint chunk ranges of one tile into k partial
+ * M x N arrays (k x M x N doubles of device memory), which a second pass sums
+ * in a fixed order before the stencil, fold, mask and dot. Shortens the longest
+ * dependency chain of products too small to fill the GPU. 0 = by the problem's
+ * shape (the default; FP64 products only), 1 = no split, 2..16 fixed (FP64 and
+ * FP32 products). Same operator, different
+ * rounding; a problem's split never depends on what it is batched with. Call
+ * before pqdts_batch_add or between solves. No counterpart in the reference. */
+int pqdts_set_hvp_split(pqdts_ctx *ctx, int split);
 
 /* ---- batched / served form (no counterpart in the reference) -------------
  * A batch advances many independent problems together: every operator of the

@@ -26,7 +26,7 @@ EXPORTS = [
     "pqdts_conv_test", "pqdts_pdot", "pqdts_paxpy", "pqdts_pgaxpy", "pqdts_pgaxpy2", "pqdts_pset",
     "pqdts_pcopy", "pqdts_pscale", "pqdts_pcopy_scale", "pqdts_transform", "pqdts_lalpha_min2",
     "pqdts_lalpha_min_auglag", "pqdts_solve", "pqdts_timers", "pqdts_simplexproj_rows",
-    "pqdts_kernel_timing", "pqdts_kernel_stats", "pqdts_info", "pqdts_set_hvp_path",
+    "pqdts_kernel_timing", "pqdts_kernel_stats", "pqdts_info", "pqdts_set_hvp_path", "pqdts_set_hvp_split",
     "pqdts_batch_create", "pqdts_batch_destroy", "pqdts_batch_admit", "pqdts_batch_step", "pqdts_batch_result",
     "pqdts_batch_release", "pqdts_batch_active", "pqdts_batch_kernel_timing", "pqdts_batch_kernel_stats",
     "pqdts_device_cu_count", "pqdts_batch_set_cu_mask",
@@ -87,6 +87,7 @@ def load():
     L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp, dp]
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
     L.pqdts_set_hvp_path.argtypes = [vp, i32]
+    L.pqdts_set_hvp_split.argtypes = [vp, i32]
     L.pqdts_batch_create.argtypes = [i32, C.POINTER(vp)]
     L.pqdts_batch_destroy.argtypes = [vp]
     L.pqdts_batch_admit.argtypes = [vp, vp, i32, i32, d, i32, i32, i32, EVENT_CB, vp, C.POINTER(i32)]
@@ -289,15 +290,19 @@ class Context:
         return conv.value, events
 
     def info(self) -> dict:
-        a = (C.c_int64 * 12)()
+        a = (C.c_int64 * 13)()
         _check(load().pqdts_info(self.h, a))
         return {"M": a[0], "N": a[1], "D": a[2], "nspan": a[3], "tile_elems": a[4], "row_tiles": a[5],
                 "col_tiles": a[6], "gram": bool(a[7]), "gram_tile_elems": a[8], "gram_nnz": a[9],
-                "ft_tile_elems": a[10], "device_bytes": a[11]}
+                "ft_tile_elems": a[10], "device_bytes": a[11], "hvp_split": a[12]}
 
     def set_hvp_path(self, path):
         """0 automatic, 1 the two F products, 2 the Gram matrix G = F^T F."""
         _check(load().pqdts_set_hvp_path(self.h, {"auto": 0, "F": 1, "G": 2}.get(path, path)))
+
+    def set_hvp_split(self, split: int):
+        """Split of the Gram product over band chunks: 0 by shape, 1 none, 2..16 fixed."""
+        _check(load().pqdts_set_hvp_split(self.h, int(split)))
 
     def kernel_timing(self, on: bool):
         _check(load().pqdts_kernel_timing(self.h, int(bool(on))))

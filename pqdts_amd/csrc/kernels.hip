@@ -69,6 +69,8 @@ struct Prob {
   double *scal;          // NSCAL device scalars
   double *cglog;         // CG_LOG_ROUNDS x 5 values logged by the device-side CG loop
   int *pn;               // NREG: number of partials the last producer of each region stored
+  int hs;                // K splits of the Gram Hessian product (1: epilogue in the product)
+  double *hw;            // hs x M x N partial products when hs > 1
 };
 
 constexpr int CG_LOG_ROUNDS = 50;
@@ -258,7 +260,7 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
 enum BMode { B_PLAIN = 0, B_EXPAND = 1, B_DIFF = 2 };
-enum EMode { E_STORE = 0, E_LOSS = 1, E_STORE_LOSS = 2, E_GRAD = 3, E_HVP = 4 };
+enum EMode { E_STORE = 0, E_LOSS = 1, E_STORE_LOSS = 2, E_GRAD = 3, E_HVP = 4, E_PART = 5 };
 enum BandSel { BAND_F = 0, BAND_FT = 1, BAND_G = 2 };
 
 struct GemmOps {
@@ -329,16 +331,23 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
   const Band &band = ops.band == BAND_F ? P.bf : (ops.band == BAND_FT ? P.bt : P.bg);
   const int N = P.N;
   const int gx = (N + TC - 1) / TC;
-  const int tile = lb / gx, n0 = (lb - tile * gx) * TC;
+  // E_PART: block lb is split `sp` of output tile lt; it multiplies band chunks
+  // [c0, c1) of the tile and stores the raw sums into partial array sp
+  const int sp = EM == E_PART ? lb / (gx * band.ntiles) : 0;
+  const int lt = lb - sp * gx * band.ntiles;
+  const int tile = lt / gx, n0 = (lt - tile * gx) * TC;
   const int r0 = tile * TR;
   const int nrows = ops.band == BAND_F ? P.D : P.M;
   const int nk = ops.band == BAND_FT ? P.D : P.M;
   const GLOBAL double *Bp = gp(P.op[ops.b]);
-  GLOBAL double *Cp = gp(P.op[ops.c]);
+  GLOBAL double *Cp = EM == E_PART ? gp(P.hw) + (int64_t)sp * P.M * N : gp(P.op[ops.c]);
   const int tid = threadIdx.x, lane = lane_id(), w = wave_id();
   const int wr0 = (w / WC) * 32, wc0 = (w % WC) * 32;
-  const GLOBAL T *ap = gp(Mf::pack(band)) + gp(band.off)[tile];
-  const int k0 = gp(band.klo)[tile], nch = gp(band.nch)[tile];
+  const int nch_t = gp(band.nch)[tile];
+  const int c0 = EM == E_PART ? (sp * nch_t) / P.hs : 0;
+  const int nch = EM == E_PART ? ((sp + 1) * nch_t) / P.hs - c0 : nch_t;
+  const GLOBAL T *ap = gp(Mf::pack(band)) + gp(band.off)[tile] + (int64_t)c0 * (KC * TR);
+  const int k0 = gp(band.klo)[tile] + c0 * KC;
 
   T ra[AE], rb[BE];
   auto gload = [&](int c) {
@@ -484,7 +493,7 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
           if (row[u] >= nrows || n >= N) continue;
           const int64_t idx = (int64_t)row[u] * N + n;
           const double v = FP32 ? acc64[i][j][q] : (double)acc[i][j][q];
-          if (EM == E_STORE || EM == E_STORE_LOSS) Cp[idx] = v;
+          if (EM == E_STORE || EM == E_STORE_LOSS || EM == E_PART) Cp[idx] = v;
           if (LOSS && fh[u]) {
             const double t = v - pv[u];
             redv = fma(t, t, redv);
@@ -514,7 +523,7 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
           if (row >= nrows || n >= N) continue;
           const int64_t idx = (int64_t)row * N + n;
           const double v = FP32 ? acc64[i][j][q] : (double)acc[i][j][q];
-          if (EM == E_STORE || EM == E_STORE_LOSS) Cp[idx] = v;
+          if (EM == E_STORE || EM == E_STORE_LOSS || EM == E_PART) Cp[idx] = v;
           if (LOSS && gp(P.fhas)[row]) {
             const double t = v - gp(P.P)[idx];
             redv = fma(t, t, redv);
@@ -892,6 +901,68 @@ __global__ __launch_bounds__(BLOCK) void kb_dot(const Prob *__restrict__ probs, 
     s = fma(xa[e], ya[e], s);
   const double b = block_sum(s, sh);
   if (threadIdx.x == 0) gp(P.part)[(int64_t)region * P.pcap + lb] = b;
+}
+
+// Epilogue of a K-split Gram Hessian product (pqdts.f90:280-330 with G = F^T F),
+// one thread per element: the hs partial products are summed in split order, then
+// bx = 2 v + the gamma stencil of dd~ (implicit column expanded), the reduced-space
+// fold, the active-set mask, and the dd.bx partial (region) - the arithmetic of the
+// product's own E_HVP epilogue. The fold value (the same expression at column
+// maxr[j], which kb_foldval_gram forms for the unsplit product) is formed here
+// from the partials of that column.
+__device__ __forceinline__ double hvp_epi_val(const Prob &P, bool red, const GLOBAL double *__restrict__ E,
+                                              const GLOBAL double *__restrict__ W, int64_t MN, int j, int n) {
+  const int M = P.M, N = P.N;
+  const int64_t e = (int64_t)j * N + n;
+  double v = W[e];
+  for (int s2 = 1; s2 < P.hs; s2++) v += W[s2 * MN + e];
+  double g = 2.0 * v;
+  if (P.gamma != 0.0) {
+    const double g2 = 2.0 * P.gamma;
+    const double e0 = stencil_src(red, E, gp(P.maxr), gp(P.ev), j, n, N);
+    if (j + 1 < M) g = g + g2 * (e0 - stencil_src(red, E, gp(P.maxr), gp(P.ev), j + 1, n, N));
+    if (j > 0) g = g - g2 * (stencil_src(red, E, gp(P.maxr), gp(P.ev), j - 1, n, N) - e0);
+  }
+  return g;
+}
+
+template <bool MASK, bool DOT>
+__global__ __launch_bounds__(BLOCK) void kb_hvp_epi(const Prob *__restrict__ probs, WorkList wl, int dd, int bx,
+                                                    int region) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  BATCH_PROLOGUE();
+  const bool red = !full;
+  const int N = P.N;
+  const int64_t MN = (int64_t)P.M * N;
+  const GLOBAL double *E = gp(P.op[dd]);
+  GLOBAL double *C = gp(P.op[bx]);
+  const GLOBAL double *W = gp(P.hw);
+  double redv = 0.0;
+  int64_t e = (int64_t)lb * EBLOCK + threadIdx.x;
+  const int64_t eend = lmin64(MN, (int64_t)(lb + 1) * EBLOCK);
+  // (j, n) of e, advanced without a 64-bit division per element
+  int j = (int)(e / N), n = (int)(e - (int64_t)j * N);
+  for (; e < eend; e += BLOCK) {
+    double g = hvp_epi_val(P, red, E, W, MN, j, n);
+    if (red) {
+      const int mr = gp(P.maxr)[j];
+      g = (n == mr) ? 0.0 : g - hvp_epi_val(P, red, E, W, MN, j, mr);
+    }
+    if (MASK && gp(P.ik)[e]) g = 0.0;
+    C[e] = g;
+    if (DOT) redv = fma(E[e], g, redv);
+    n += BLOCK;
+    if (n >= N) {
+      const int q = n / N;
+      j += q;
+      n -= q * N;
+    }
+  }
+  if (DOT) {
+    const double b = block_sum(redv, sh);
+    if (threadIdx.x == 0) gp(P.part)[(int64_t)region * P.pcap + lb] = b;
+    if (threadIdx.x == 0 && lb == 0) gp(P.pn)[region] = nblk;
+  }
 }
 
 // BLAS-1 of the reference (pqdts.f90:485-587): z = beta*y + alpha*x and its

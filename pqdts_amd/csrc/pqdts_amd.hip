@@ -151,6 +151,10 @@ struct pqdts_ctx {
   bool gram_auto = false;
   int64_t gram_elems = 0, gram_nnz = 0;
   std::vector<int> glo, gw;
+  int hs = 1;                // split of the Gram product over band chunks (Prob::hs)
+  int hs_pref = 0;           // 0: by shape (apply_gram_split), else fixed
+  int gram_nch = 0;          // most band chunks of one G tile
+  double *hw = nullptr;      // hs x M x N partial products when hs > 1
   bool use_gram() const { return hvp_pref == 2 || (hvp_pref == 0 && gram_auto); }
   std::unique_ptr<Engine> eng;  // standalone engine (this problem alone, on stream s)
   Engine &engine();
@@ -200,6 +204,8 @@ void refresh_desc(pqdts_ctx *c) {
   p.part = c->part;
   p.scal = c->scal;
   p.pn = c->pn;
+  p.hs = c->hs;
+  p.hw = c->hw;
   HIPC(hipMemcpyAsync(c->ddesc, &c->desc, sizeof(Prob), hipMemcpyHostToDevice, c->s));
   HIPC(hipStreamSynchronize(c->s));
 }
@@ -540,7 +546,17 @@ struct Engine {
       const bool gram = ctx[p]->use_gram();
       (gram ? (fp32[p] ? g32 : g64) : (fp32[p] ? f32 : f64)).push_back(p);
     }
-    // Gram path: fold values, then one product 2 G p~ with the epilogue
+    // Gram path: fold values, then one product 2 G p~ with the epilogue.
+    // Problems with a split product write partials instead; their summing pass
+    // forms the fold values itself.
+    std::vector<int> g64s, g32s;
+    auto split_off = [&](std::vector<int> &v, std::vector<int> &sv) {
+      std::vector<int> keep;
+      for (int p : v) (ctx[p]->hs > 1 && (!fp32[p] || ctx[p]->hs_pref > 0) ? sv : keep).push_back(p);
+      v.swap(keep);
+    };
+    split_off(g64, g64s);
+    split_off(g32, g32s);
     std::vector<int> G = g64;
     G.insert(G.end(), g32.begin(), g32.end());
     std::vector<int> GR = reduced_of(G);
@@ -554,8 +570,23 @@ struct Engine {
     }
     const GemmOps hg{BAND_G, dd, bx, dd, dd, R_B};
     // B = p with its implicit columns expanded (reduced problems)
+    std::vector<int> GS;
     hvp_gemm<double, B_EXPAND>(g64, hg, mask, dot, "hvp_G");
     hvp_gemm<float, B_EXPAND>(g32, hg, mask, dot, "hvp_G32");
+    part_gemm<double>(g64s, hg, "hvp_Gs");
+    part_gemm<float>(g32s, hg, "hvp_G32s");
+    GS = g64s;
+    GS.insert(GS.end(), g32s.begin(), g32s.end());
+    if (!GS.empty()) {
+      hipEvent_t e0 = kbegin();
+      launch(GS, [&](int p) { return nb_elem(p); }, [&](const WorkList &wl, int tot) {
+        if (mask && dot)
+          kb_hvp_epi<true, true><<<tot, BLOCK, 0, s>>>(dprobs, wl, dd, bx, R_B);
+        else
+          kb_hvp_epi<false, false><<<tot, BLOCK, 0, s>>>(dprobs, wl, dd, bx, R_B);
+      });
+      kend(e0, "hvp_epi", GS, [&](int p) { return stream_work(p, 8.0 * (ctx[p]->hs + 3) + 1.0); });
+    }
     // F path: O = F p~, fold values, then 2 F^T O with the epilogue
     for (int pass = 0; pass < 2; pass++) {
       const std::vector<int> &FS = pass ? f32 : f64;
@@ -574,6 +605,16 @@ struct Engine {
       else
         hvp_gemm<double, B_PLAIN>(FS, ht, mask, dot, "spmmT_F");
     }
+  }
+  // split Gram product: hs partial products per problem (nothing fused)
+  template <typename T>
+  void part_gemm(const std::vector<int> &S, GemmOps ops, const char *kname) {
+    if (S.empty()) return;
+    hipEvent_t e0 = kbegin();
+    launch(S, [&](int p) { return nb_gemm(p, ops.band) * ctx[p]->hs; }, [&](const WorkList &wl, int tot) {
+      launch_gemm<T, B_EXPAND, E_PART, false, false>(tot, s, dprobs, wl, ops);
+    });
+    kend(e0, kname, S, [&](int p) { return gemm_work(p, ops.band, sizeof(T) == 4); });
   }
   template <typename T, int BM>
   void hvp_gemm(const std::vector<int> &S, GemmOps ops, bool mask, bool dot, const char *kname) {
@@ -1058,6 +1099,8 @@ void ensure_fp32(pqdts_ctx *c) {
   if (c->eng) c->eng->attach(0, c);
 }
 
+void apply_gram_split(pqdts_ctx *c);
+
 // Forms G = F^T F once, on the device: F is expanded to a dense D x M matrix and
 // multiplied by the packed F^T band with the band GEMM itself; the rows of G are
 // then packed into band tiles like F.
@@ -1098,6 +1141,8 @@ void ensure_gram(pqdts_ctx *c) {
         M, [&](int k) { return std::make_pair(c->glo[k], c->gw[k]); },
         [&](int k, int l) { return Gh[(size_t)k * M + l]; });
     upload_band(c, c->bg, h);
+    c->gram_nch = 0;
+    for (int n : h.nch) c->gram_nch = std::max(c->gram_nch, n);
   } catch (...) {
     if (Fd) (void)hipFree(Fd);
     if (Gd) (void)hipFree(Gd);
@@ -1106,6 +1151,42 @@ void ensure_gram(pqdts_ctx *c) {
   (void)hipFree(Fd);
   (void)hipFree(Gd);
   if (c->bf.pack32) ensure_fp32(c);
+  apply_gram_split(c);
+}
+
+// Split of the Gram product over the band chunks of each tile (Prob::hs). A
+// product of fewer tiles than the chip holds at once (about 4 workgroups per CU)
+// runs at the latency of its longest tile, the one whose band spans the most
+// chunks; splitting the chunks over hs workgroups shortens that chain hs-fold at
+// the price of hs partial results in HBM and one summing pass (which also forms
+// the fold values, so the separate fold-value pass goes away). By shape: the
+// largest hs in {2, 4} with hs x tiles <= 1024 and at least 6 chunks per slice
+// of the longest tile; FP64 products only: the FP32 products of mixed-precision
+// solves run batched, where the chip is already full and the summing pass costs
+// more than the shorter chain saves. An explicit split applies to both. The
+// split depends on the problem and its precision alone, so a problem computes
+// the same bits alone and in a batch.
+void apply_gram_split(pqdts_ctx *c) {
+  int hs = 1;
+  if (c->hs_pref > 0) {
+    hs = c->hs_pref;
+  } else if (c->bg.pack) {
+    const int ntile = cdiv(c->N, TILE) * c->bg.ntiles, mx = c->gram_nch;
+    for (int k = 2; k <= 4; k *= 2)
+      if (k * ntile <= 1024 && mx >= 6 * k) hs = k;
+  }
+  // the partials are only needed once G exists
+  const bool need = hs > 1 && c->bg.pack;
+  if (hs != c->hs || need != (c->hw != nullptr)) {
+    if (c->hw) {
+      HIPC(hipStreamSynchronize(c->s));
+      (void)hipFree(c->hw);
+    }
+    c->hw = nullptr;
+    c->hs = 1;
+    if (need) c->hw = dalloc<double>((int64_t)hs * c->MN);
+    c->hs = hs;
+  }
   refresh_desc(c);
   if (c->eng) c->eng->attach(0, c);
 }
@@ -1119,7 +1200,7 @@ void destroy_ctx(pqdts_ctx *c) {
   free_band(c->bg);
   void *ptrs[] = {c->flo,       c->flen,     c->foff, c->fval, c->fhas, c->kd_lo, c->kd_hi, c->P,
                   c->O,         c->diag_full, c->diag_red, c->maxr, c->ik,  c->ev,    c->gm,    c->lam,
-                  c->lmin,      c->part,     c->scal, c->ddesc, c->pn};
+                  c->lmin,      c->part,     c->scal, c->ddesc, c->pn, c->hw};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (double *p : c->slot)
@@ -1699,7 +1780,9 @@ int pqdts_info(pqdts_ctx *ctx, int64_t *info) {
     info[10] = ctx->bt.elems;
     // device bytes held by this problem
     info[11] = 8 * (6 * ctx->MN + 2 * ctx->DN + ctx->nspan + ctx->bf.elems + ctx->bt.elems + ctx->bg.elems) +
-               4 * (ctx->bf.pack32 ? ctx->bf.elems + ctx->bt.elems + ctx->bg.elems : 0) + ctx->MN;
+               4 * (ctx->bf.pack32 ? ctx->bf.elems + ctx->bt.elems + ctx->bg.elems : 0) + ctx->MN +
+               (ctx->hw ? 8 * ctx->hs * ctx->MN : 0);
+    info[12] = ctx->hs;
   });
 }
 
@@ -1709,6 +1792,15 @@ int pqdts_set_hvp_path(pqdts_ctx *ctx, int path) {
     require(path >= 0 && path <= 2, "hvp path must be 0 (auto), 1 (F products) or 2 (Gram)");
     ctx->hvp_pref = path;
     if (ctx->use_gram()) ensure_gram(ctx);
+  });
+}
+
+int pqdts_set_hvp_split(pqdts_ctx *ctx, int split) {
+  return guarded([&] {
+    checked(ctx);
+    require(split >= 0 && split <= 16, "split must be 0 (by shape) or 1..16");
+    ctx->hs_pref = split;
+    apply_gram_split(ctx);
   });
 }
 

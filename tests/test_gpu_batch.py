@@ -29,8 +29,10 @@ def start(ctx, prob):
     ctx.put("X", np.full((prob.M, prob.N), 1.0 / prob.N))
 
 
-def solve_alone(prob, maxiter=300):
+def solve_alone(prob, maxiter=300, setup=None):
     ctx = pqdts_amd.Context.from_problem(prob)
+    if setup:
+        setup(ctx)
     start(ctx, prob)
     conv, ev = ctx.solve(2, maxiter, prob.eps, 1, 0)
     return ctx.get("X"), conv, [e for e in ev]
@@ -55,6 +57,35 @@ def test_batch_equals_alone_bitwise():
         evb = b.events(i)
         assert [k for k, _ in evb] == [k for k, _ in eva]
         assert all(np.array_equal(x[1], y[1]) for x, y in zip(evb, eva))
+
+
+def test_split_gram_batch_equals_alone_bitwise():
+    """Problems whose Gram product is split over band chunks (partials summed by a
+    second pass) beside unsplit ones: still bit-identical alone and batched."""
+    probs = problems()
+    splits = [3, 1, 2, 0]
+
+    def setup_of(i):
+        def setup(c):
+            c.set_hvp_path("G")
+            c.set_hvp_split(splits[i])
+        return setup
+
+    alone = [solve_alone(p, setup=setup_of(i)) for i, p in enumerate(probs)]
+    b = pqdts_amd.Batch(8)
+    ctxs = []
+    for i, p in enumerate(probs):
+        c = pqdts_amd.Context.from_problem(p)
+        setup_of(i)(c)
+        assert c.info()["hvp_split"] == (splits[i] or c.info()["hvp_split"])
+        start(c, p)
+        ctxs.append(c)
+        b.admit(c, 2, 300, p.eps, 1, 0, 0)
+    res = b.run()
+    for i, (p, c) in enumerate(zip(probs, ctxs)):
+        Xa, conva, _ = alone[i]
+        assert np.array_equal(Xa, c.get("X")), f"problem {p.name}"
+        assert res[i]["conv"] == conva
 
 
 def test_continuous_admission_matches_alone():

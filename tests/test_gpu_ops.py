@@ -67,12 +67,16 @@ def test_gpu_visible():
     assert pqdts_amd.device_count() >= 1
 
 
-@pytest.mark.parametrize("path", ["F", "G"])
+@pytest.mark.parametrize("path", ["F", "G", "G3"])
 @pytest.mark.parametrize("full", [True, False])
 def test_L_dL_BF(pair, full, path):
-    """path F: BF as 2 F^T (F p); path G: BF with the banded Gram matrix F^T F."""
+    """path F: BF as 2 F^T (F p); path G: BF with the banded Gram matrix F^T F;
+    G3: the Gram product split three ways over band chunks."""
     ctx, orc, M, N = pair
-    ctx.set_hvp_path(path)
+    ctx.set_hvp_path(path[0])
+    if path == "G3":
+        ctx.set_hvp_split(3)
+        assert ctx.info()["hvp_split"] == 3
     rng = np.random.default_rng(1)
     set_both(ctx, orc, "X", simplex_rows(rng, M, N))
     if not full:
@@ -95,7 +99,7 @@ def test_L_dL_BF(pair, full, path):
     ctx.BF("XTMP", "HP")
     orc.BF("XTMP", "HP")
     close_arr(ctx.get("HP"), orc.buf("HP"))
-    assert ctx.info()["gram"] == (path == "G")
+    assert ctx.info()["gram"] == (path[0] == "G")
     orc.Bdiag()
     close_arr(ctx.Bdiag(), orc.diag)
 

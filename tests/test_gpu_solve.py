@@ -57,13 +57,15 @@ def test_trajectory_prefix_matches_reference(cfg1, path):
             assert _rel(a[2], b[2]) <= 1e-12 and _rel(a[4], b[4]) <= 1e-9
 
 
-@pytest.mark.parametrize("path", ["F", "G"])
+@pytest.mark.parametrize("path", ["F", "G", "G4"])
 def test_tight_endpoint_matches_reference(cfg1, path):
     z = np.load(os.path.join(GOLD, "cfg1_ref_states.npz"))
     X1, X4 = z["final_eps1e-11_t1"], z["final_eps1e-11_t4"]
     spread = float(np.max(np.abs(X1 - X4)))
     ctx = pqdts_amd.Context.from_problem(cfg1)
-    ctx.set_hvp_path(path)
+    ctx.set_hvp_path(path[0])
+    if path == "G4":
+        ctx.set_hvp_split(4)
     ctx.put("X", np.full((cfg1.M, cfg1.N), 1.0 / cfg1.N))
     conv, ev = ctx.solve(2, 2000, 1e-11, 1, 0)
     X = ctx.get("X")
