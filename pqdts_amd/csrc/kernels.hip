@@ -71,6 +71,7 @@ struct Prob {
   int *pn;               // NREG: number of partials the last producer of each region stored
   int hs;                // K splits of the Gram Hessian product (1: epilogue in the product)
   double *hw;            // hs x M x N partial products when hs > 1
+  float *p32;            // FP32 copy of the CG direction, implicit column expanded (mixed precision), or null
 };
 
 constexpr int CG_LOG_ROUNDS = 50;
@@ -259,7 +260,10 @@ __device__ __forceinline__ double stencil_src(bool exp, const GLOBAL double *__r
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
-enum BMode { B_PLAIN = 0, B_EXPAND = 1, B_DIFF = 2 };
+// B operand: as stored, with the implicit column expanded (reduced problems), minus
+// P, or the FP32 expanded copy of the CG direction (P.p32, written where the
+// direction is formed)
+enum BMode { B_PLAIN = 0, B_EXPAND = 1, B_DIFF = 2, B_P32 = 3 };
 enum EMode { E_STORE = 0, E_LOSS = 1, E_STORE_LOSS = 2, E_GRAD = 3, E_HVP = 4, E_PART = 5 };
 enum BandSel { BAND_F = 0, BAND_FT = 1, BAND_G = 2 };
 
@@ -360,7 +364,9 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
       const int kk = e / TC, cc = e - kk * TC;
       const int k = k0 + c * KC + kk, n = n0 + cc;
       double v = 0.0;
-      if (k < nk && n < N) {
+      if (BM == B_P32 && k < nk && n < N) {
+        v = (double)gp(P.p32)[(int64_t)k * N + n];
+      } else if (k < nk && n < N) {
         const int64_t idx = (int64_t)k * N + n;
         v = Bp[idx];
         if (BM == B_DIFF) v -= gp(P.P)[idx];
@@ -997,6 +1003,17 @@ __global__ __launch_bounds__(BLOCK) void kb_axpby(const Prob *__restrict__ probs
 // Active set + CG start (pqdts.f90:1986-2031): ik = X < tol and G > 0, G[ik] = 0;
 // partials of nact (region 0), |G|^2 (region 1) and r.z (region 2); r = -G;
 // z = r / sqrt(diag); p = z; sn = 0; ev[j] = -(sum p[j,:] - p[j,maxr]) when reduced.
+// FP32 copy of row j of the CG direction (operand 3) with its implicit column
+// (mr, -1 in full space) replaced by evj: the B operand of the mixed-precision
+// Hessian product, rounded once here instead of in every tile that reads it.
+// Each lane re-reads the entries it has just written.
+__device__ __forceinline__ void store_p32_row(const Prob &P, int j, int mr, double evj) {
+  const int64_t base = (int64_t)j * P.N;
+  const GLOBAL double *p = gp(P.op[3]) + base;
+  GLOBAL float *d = gp(P.p32) + base;
+  for (int n = lane_id(); n < P.N; n += WAVE) d[n] = (float)(n == mr ? evj : p[n]);
+}
+
 __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ probs, WorkList wl, int tactive) {
   __shared__ double sh[WAVES_PER_BLOCK];
   ROW_PROLOGUE();
@@ -1027,11 +1044,14 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
       psum += zz;
       if (n == mr) own = zz;
     }
+    double evj = 0.0;
     if (!full) {
       psum = wave_sum(psum);
       own = wave_sum(own);
-      if (lane_id() == 0) gp(P.ev)[j] = -(psum - own);
+      evj = -(psum - own);
+      if (lane_id() == 0) gp(P.ev)[j] = evj;
     }
+    if (P.p32) store_p32_row(P, j, mr, evj);
   }
   double t = block_sum(nact, sh);
   if (threadIdx.x == 0) gp(P.part)[lb] = t;
@@ -1074,11 +1094,14 @@ __device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, doub
     psum += p;
     if (n == mr) own = p;
   }
+  double evj = 0.0;
   if (!full) {
     psum = wave_sum(psum);
     own = wave_sum(own);
-    if (lane_id() == 0) gp(P.ev)[j] = -(psum - own);
+    evj = -(psum - own);
+    if (lane_id() == 0) gp(P.ev)[j] = evj;
   }
+  if (P.p32) store_p32_row(P, j, mr, evj);
 }
 
 // CG update (pqdts.f90:2060-2070). Every block first sums the Hessian product's

@@ -155,6 +155,7 @@ struct pqdts_ctx {
   int hs_pref = 0;           // 0: by shape (apply_gram_split), else fixed
   int gram_nch = 0;          // most band chunks of one G tile
   double *hw = nullptr;      // hs x M x N partial products when hs > 1
+  float *p32 = nullptr;      // FP32 expanded CG direction (mixed precision; Prob::p32)
   bool use_gram() const { return hvp_pref == 2 || (hvp_pref == 0 && gram_auto); }
   std::unique_ptr<Engine> eng;  // standalone engine (this problem alone, on stream s)
   Engine &engine();
@@ -290,6 +291,8 @@ struct Engine {
     ctx[slot] = c;
     Prob d = c->desc;
     d.cglog = dlog + (int64_t)slot * CG_LOG_ROUNDS * 5;
+    // the CG kernels write the FP32 direction copy only for mixed-precision slots
+    d.p32 = fp32[slot] ? c->p32 : nullptr;
     if (!private_scal) {
       if (!dscal) dscal = dalloc<double>((int64_t)NSCAL * cap);
       d.scal = dscal + (int64_t)NSCAL * slot;
@@ -533,8 +536,10 @@ struct Engine {
     gemm_t<double, B_DIFF, E_GRAD, false, false>(S, GemmOps{BAND_FT, OP_O, g, x, 0, 0}, "spmmT_F");
   }
 
-  // BF (pqdts.f90:246-342): bx = H dd; with `dot` the partial dd.bx goes to S_V2
-  void BF(const std::vector<int> &S, int dd, int bx, bool have_ev, bool mask, bool dot) {
+  // BF (pqdts.f90:246-342): bx = H dd; with `dot` the partial dd.bx goes to S_V2.
+  // p32: dd is the CG direction, whose FP32 expanded copy the CG kernels keep
+  // (Prob::p32) for the mixed-precision problems of S.
+  void BF(const std::vector<int> &S, int dd, int bx, bool have_ev, bool mask, bool dot, bool p32 = false) {
     struct Scope sc(this, "BF");
     need_maxr(S);
     std::vector<int> R = reduced_of(S);
@@ -572,9 +577,15 @@ struct Engine {
     // B = p with its implicit columns expanded (reduced problems)
     std::vector<int> GS;
     hvp_gemm<double, B_EXPAND>(g64, hg, mask, dot, "hvp_G");
-    hvp_gemm<float, B_EXPAND>(g32, hg, mask, dot, "hvp_G32");
-    part_gemm<double>(g64s, hg, "hvp_Gs");
-    part_gemm<float>(g32s, hg, "hvp_G32s");
+    if (p32)
+      hvp_gemm<float, B_P32>(g32, hg, mask, dot, "hvp_G32");
+    else
+      hvp_gemm<float, B_EXPAND>(g32, hg, mask, dot, "hvp_G32");
+    part_gemm<double, B_EXPAND>(g64s, hg, "hvp_Gs");
+    if (p32)
+      part_gemm<float, B_P32>(g32s, hg, "hvp_G32s");
+    else
+      part_gemm<float, B_EXPAND>(g32s, hg, "hvp_G32s");
     GS = g64s;
     GS.insert(GS.end(), g32s.begin(), g32s.end());
     if (!GS.empty()) {
@@ -591,7 +602,9 @@ struct Engine {
     for (int pass = 0; pass < 2; pass++) {
       const std::vector<int> &FS = pass ? f32 : f64;
       if (FS.empty()) continue;
-      if (pass)
+      if (pass && p32)
+        gemm_t<float, B_P32, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F32");
+      else if (pass)
         gemm_t<float, B_EXPAND, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F32");
       else
         gemm_t<double, B_EXPAND, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F");
@@ -607,12 +620,12 @@ struct Engine {
     }
   }
   // split Gram product: hs partial products per problem (nothing fused)
-  template <typename T>
+  template <typename T, int BM>
   void part_gemm(const std::vector<int> &S, GemmOps ops, const char *kname) {
     if (S.empty()) return;
     hipEvent_t e0 = kbegin();
     launch(S, [&](int p) { return nb_gemm(p, ops.band) * ctx[p]->hs; }, [&](const WorkList &wl, int tot) {
-      launch_gemm<T, B_EXPAND, E_PART, false, false>(tot, s, dprobs, wl, ops);
+      launch_gemm<T, BM, E_PART, false, false>(tot, s, dprobs, wl, ops);
     });
     kend(e0, kname, S, [&](int p) { return gemm_work(p, ops.band, sizeof(T) == 4); });
   }
@@ -941,7 +954,7 @@ void Engine::tick() {
       int ib1 = S_B1;
       for (int j = 1; j <= 50; j++) {
         kround = j;
-        BF(A2, XT, HP, true, true, true);
+        BF(A2, XT, HP, true, true, true, true);
         const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
         cg_update(A2, ib1);
         cg_dir(A2, ib1, i_rz, j);
@@ -1094,6 +1107,7 @@ void ensure_fp32(pqdts_ctx *c) {
     k_round_f32<<<std::min(4096, cdiv(b->elems, 256)), 256, 0, c->s>>>(b->pack, b->pack32, b->elems);
     HIPC(hipGetLastError());
   }
+  if (!c->p32) c->p32 = dalloc<float>(c->MN);
   HIPC(hipStreamSynchronize(c->s));
   refresh_desc(c);
   if (c->eng) c->eng->attach(0, c);
@@ -1200,7 +1214,7 @@ void destroy_ctx(pqdts_ctx *c) {
   free_band(c->bg);
   void *ptrs[] = {c->flo,       c->flen,     c->foff, c->fval, c->fhas, c->kd_lo, c->kd_hi, c->P,
                   c->O,         c->diag_full, c->diag_red, c->maxr, c->ik,  c->ev,    c->gm,    c->lam,
-                  c->lmin,      c->part,     c->scal, c->ddesc, c->pn, c->hw};
+                  c->lmin,      c->part,     c->scal, c->ddesc, c->pn, c->hw, c->p32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (double *p : c->slot)
@@ -1781,7 +1795,7 @@ int pqdts_info(pqdts_ctx *ctx, int64_t *info) {
     // device bytes held by this problem
     info[11] = 8 * (6 * ctx->MN + 2 * ctx->DN + ctx->nspan + ctx->bf.elems + ctx->bt.elems + ctx->bg.elems) +
                4 * (ctx->bf.pack32 ? ctx->bf.elems + ctx->bt.elems + ctx->bg.elems : 0) + ctx->MN +
-               (ctx->hw ? 8 * ctx->hs * ctx->MN : 0);
+               (ctx->hw ? 8 * ctx->hs * ctx->MN : 0) + (ctx->p32 ? 4 * ctx->MN : 0);
     info[12] = ctx->hs;
   });
 }
@@ -1883,6 +1897,7 @@ int pqdts_batch_admit(pqdts_batch *b, pqdts_ctx *ctx, int mode, int maxiter, dou
     require(p >= 0, "batch is full");
     HIPC(hipStreamSynchronize(ctx->s));
     if (precision) ensure_fp32(ctx);
+    e.fp32[p] = precision;
     e.attach(p, ctx);
     PState *q = new PState();
     q->mode = mode;
@@ -1893,7 +1908,6 @@ int pqdts_batch_admit(pqdts_batch *b, pqdts_ctx *ctx, int mode, int maxiter, dou
     q->cb = cb;
     q->user = user;
     e.st[p].reset(q);
-    e.fp32[p] = precision;
     ctx->full = true;
     if (slot) *slot = p;
   });

@@ -88,6 +88,31 @@ def test_split_gram_batch_equals_alone_bitwise():
         assert res[i]["conv"] == conva
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_launch_forms_same_bits(precision):
+    """FP64 band products of launches too small to fill the GPU use the latency
+    form of the epilogue, large launches the occupancy form. A small problem
+    batched beside a large one (every product launch then has over 1024
+    workgroups) computes the same bits as alone, in FP64 and mixed precision."""
+    small = workloads.random_problem(130, 60, 260, 0.6, 1e-2, 3e6, seed=13, eps=1e-7, name="small")
+    big = workloads.random_problem(3200, 1400, 6400, 0.8, 1e-3, 1e7, seed=14, eps=1e-7, name="big")
+
+    def run(probs):
+        b = pqdts_amd.Batch(4)
+        ctxs = []
+        for p in probs:
+            c = pqdts_amd.Context.from_problem(p)
+            start(c, p)
+            ctxs.append(c)
+            b.admit(c, 2, 2, p.eps, 1, 0, precision)
+        b.run()
+        return [c.get("X") for c in ctxs]
+
+    (xa,) = run([small])
+    xb, _ = run([small, big])
+    assert np.array_equal(xa, xb), f"max diff {np.max(np.abs(xa - xb)):.3e}"
+
+
 def test_continuous_admission_matches_alone():
     probs = problems()
     alone = [solve_alone(p) for p in probs]
