@@ -72,6 +72,7 @@ struct Prob {
   int hs;                // K splits of the Gram Hessian product (1: epilogue in the product)
   double *hw;            // hs x M x N partial products when hs > 1
   float *p32;            // FP32 copy of the CG direction, implicit column expanded (mixed precision), or null
+  double *pb;            // F_LATE: per row, sum of the direction over the free entries outside maxr
 };
 
 constexpr int CG_LOG_ROUNDS = 50;
@@ -82,6 +83,9 @@ constexpr int NSCAL = 16 + 3 * LS_K;   // solver scalars, then per-trial objecti
 
 // flags of a problem within one launch
 constexpr unsigned F_FULL = 1u;  // full space (tfullspace); otherwise reduced
+// Mixed-precision CG of a reduced problem: the Hessian product stores its result
+// unfolded with the fold value of each row (gm), and kb_cg_update applies the fold
+constexpr unsigned F_LATE = 2u;
 
 struct WorkList {
   int np;
@@ -107,14 +111,19 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 __device__ __forceinline__ int wave_id() { return threadIdx.x / WAVE; }
 
 // problem slot i of the work list that owns flat block `bid`
+// Entry of block bid: the last i with boff[i] <= bid (boff is nondecreasing from
+// boff[0] = 0), i.e. the number of such entries minus one. Every lane tests
+// MAXB / WAVE entries and a ballot counts them: one round of independent loads
+// instead of a binary search's chain of dependent ones. Wave-uniform result.
 __device__ __forceinline__ int wl_find(const WorkList &wl, int bid) {
-  int lo = 0, hi = wl.np;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (wl.boff[mid] <= bid) lo = mid;
-    else hi = mid;
+  const int l = lane_id();
+  int cnt = 0;
+#pragma unroll
+  for (int b0 = 0; b0 < MAXB; b0 += WAVE) {
+    const int i = b0 + l;
+    cnt += __popcll(__ballot(i < wl.np && wl.boff[i] <= bid));
   }
-  return lo;
+  return cnt - 1;
 }
 
 #define BATCH_PROLOGUE()                                   \
@@ -125,6 +134,8 @@ __device__ __forceinline__ int wl_find(const WorkList &wl, int bid) {
   const int nblk = wl.boff[wi_ + 1] - wl.boff[wi_];        \
   const bool full = (wl.flags[wi_] & F_FULL) != 0;         \
   const double arg = wl.a[wi_];                            \
+  const bool late = (wl.flags[wi_] & F_LATE) != 0;         \
+  (void)late;                                              \
   (void)full;                                              \
   (void)nblk;                                              \
   (void)arg
@@ -324,7 +335,7 @@ struct MfmaT<float> {
 constexpr int GT = 64;  // rows and columns of an output tile
 
 template <typename T, int KC, int BM, int EM, bool MASK, bool DOT, bool LAT>
-__device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops, int lb, bool full,
+__device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops, int lb, bool full, bool late,
                                                T (*As)[KC * GT], T (*Bs)[KC * GT], double &redv) {
   constexpr int TR = GT, TC = GT, WC = 2, MR = 2, MC = 2;
   constexpr int AE = KC * TR / BLOCK;
@@ -448,6 +459,8 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
   const bool sten = GH && P.gamma != 0.0;
   const GLOBAL double *__restrict__ E = GH ? gp(P.op[ops.e]) : nullptr;
   const GLOBAL double *__restrict__ dd = DOT ? gp(P.op[ops.dd]) : nullptr;
+  // the late fold exists for the masked FP32 Hessian product with its dot (CG)
+  constexpr bool LATE_OK = FP32 && EM == E_HVP && MASK && DOT;
   if constexpr (!FP32 && LAT) {
 #pragma unroll
   for (int i = 0; i < MR; i++)
@@ -541,10 +554,25 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
               if (row + 1 < nrows) g = g + g2 * (e0 - stencil_src(exp, E, gp(P.maxr), gp(P.ev), row + 1, n, N));
               if (row > 0) g = g - g2 * (stencil_src(exp, E, gp(P.maxr), gp(P.ev), row - 1, n, N) - e0);
             }
-            if (red) g = (n == gp(P.maxr)[row]) ? 0.0 : g - gp(P.gm)[row];
-            if (MASK && gp(P.ik)[idx]) g = 0.0;
-            Cp[idx] = g;
-            if (DOT) redv = fma(dd[idx], g, redv);
+            if (LATE_OK && late && red) {
+              // fold left to kb_cg_update: store g (0 where masked) and, at the
+              // implicit column, the row's fold value; the dot uses
+              // sum_{n != maxr, free} p (g - gm) = sum p g - gm sum p
+              const int mr = gp(P.maxr)[row];
+              const bool m = gp(P.ik)[idx] != 0;
+              if (n == mr) {
+                gp(P.gm)[row] = g;
+                redv = fma(-g, gp(P.pb)[row], redv);
+              } else if (!m) {
+                redv = fma(dd[idx], g, redv);
+              }
+              Cp[idx] = m ? 0.0 : g;
+            } else {
+              if (red) g = (n == gp(P.maxr)[row]) ? 0.0 : g - gp(P.gm)[row];
+              if (MASK && gp(P.ik)[idx]) g = 0.0;
+              Cp[idx] = g;
+              if (DOT) redv = fma(dd[idx], g, redv);
+            }
           }
         }
   }
@@ -557,7 +585,7 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
   __shared__ double sh[WAVES_PER_BLOCK];
   BATCH_PROLOGUE();
   double redv = 0.0;
-  band_gemm_tile<T, KC, BM, EM, MASK, DOT, LAT>(P, ops, lb, full, As, Bs, redv);
+  band_gemm_tile<T, KC, BM, EM, MASK, DOT, LAT>(P, ops, lb, full, late, As, Bs, redv);
   if (EM == E_LOSS || EM == E_STORE_LOSS || DOT) {
     const double b = block_sum(redv, sh);
     if (threadIdx.x == 0) gp(P.part)[(int64_t)ops.region * P.pcap + lb] = b;
@@ -1007,11 +1035,23 @@ __global__ __launch_bounds__(BLOCK) void kb_axpby(const Prob *__restrict__ probs
 // (mr, -1 in full space) replaced by evj: the B operand of the mixed-precision
 // Hessian product, rounded once here instead of in every tile that reads it.
 // Each lane re-reads the entries it has just written.
-__device__ __forceinline__ void store_p32_row(const Prob &P, int j, int mr, double evj) {
+// With `late` (F_LATE, reduced problems) also pb[j] = sum of p[j,n] over the free
+// entries (ik = 0) outside the implicit column, for the Hessian product's dot.
+__device__ __forceinline__ void store_p32_row(const Prob &P, int j, int mr, double evj, bool late) {
   const int64_t base = (int64_t)j * P.N;
   const GLOBAL double *p = gp(P.op[3]) + base;
+  const GLOBAL uint8_t *ik = gp(P.ik) + base;
   GLOBAL float *d = gp(P.p32) + base;
-  for (int n = lane_id(); n < P.N; n += WAVE) d[n] = (float)(n == mr ? evj : p[n]);
+  double b = 0.0;
+  for (int n = lane_id(); n < P.N; n += WAVE) {
+    const double v = p[n];
+    d[n] = (float)(n == mr ? evj : v);
+    if (late && n != mr && !ik[n]) b += v;
+  }
+  if (late) {
+    b = wave_sum(b);
+    if (lane_id() == 0) gp(P.pb)[j] = b;
+  }
 }
 
 __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ probs, WorkList wl, int tactive) {
@@ -1051,7 +1091,7 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
       evj = -(psum - own);
       if (lane_id() == 0) gp(P.ev)[j] = evj;
     }
-    if (P.p32) store_p32_row(P, j, mr, evj);
+    if (P.p32) store_p32_row(P, j, mr, evj, late && !full);
   }
   double t = block_sum(nact, sh);
   if (threadIdx.x == 0) gp(P.part)[lb] = t;
@@ -1064,14 +1104,21 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
 
 // One row j of the CG update, one wave: sn += a p; r += (-a) Hp; z = r / sqrt(diag);
 // r.z and r.r accumulate into rz, rr (pqdts.f90:2060-2070).
-__device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, double a, double &rz, double &rr) {
+// With `lf` (F_LATE in reduced space) Hp is folded here: 0 at the implicit column
+// and on the active set, Hp - gm[j] elsewhere.
+__device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, bool lf, double a, double &rz,
+                                              double &rr) {
   const int N = P.N;
   const double sq = sqrt(full ? gp(P.diag_full)[j] : gp(P.diag_red)[j]);
+  const int mr = lf ? gp(P.maxr)[j] : -1;
+  const double gmj = lf ? gp(P.gm)[j] : 0.0;
   const int64_t base = (int64_t)j * N;
   for (int n = lane_id(); n < N; n += WAVE) {
     const int64_t idx = base + n;
     gp(P.op[4])[idx] = gp(P.op[4])[idx] + a * gp(P.op[3])[idx];
-    const double r = gp(P.op[1])[idx] + (-a) * gp(P.op[2])[idx];
+    double hp = gp(P.op[2])[idx];
+    if (lf) hp = (n == mr || gp(P.ik)[idx]) ? 0.0 : hp - gmj;
+    const double r = gp(P.op[1])[idx] + (-a) * hp;
     gp(P.op[1])[idx] = r;
     const double zz = r / sq;
     gp(P.op[5])[idx] = zz;
@@ -1082,7 +1129,7 @@ __device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, d
 
 // One row j of the new direction, one wave: p = beta p + 1 z (pqdts.f90:2075) and,
 // in reduced space, ev[j] = -(sum p[j,:] - p[j,maxr]) for the next product.
-__device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, double beta) {
+__device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, bool late, double beta) {
   const int N = P.N;
   const int mr = full ? -1 : gp(P.maxr)[j];
   double psum = 0.0, own = 0.0;
@@ -1101,7 +1148,7 @@ __device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, doub
     evj = -(psum - own);
     if (lane_id() == 0) gp(P.ev)[j] = evj;
   }
-  if (P.p32) store_p32_row(P, j, mr, evj);
+  if (P.p32) store_p32_row(P, j, mr, evj, late && !full);
 }
 
 // CG update (pqdts.f90:2060-2070). Every block first sums the Hessian product's
@@ -1117,7 +1164,7 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_update(const Prob *__restrict__ p
   if (threadIdx.x == 0 && lb == 0) gp(P.scal)[i_v2] = v2;
   const double a = gp(P.scal)[i_b1] / v2;
   double rz = 0.0, rr = 0.0;
-  if (j < P.M) cg_update_row(P, j, full, a, rz, rr);
+  if (j < P.M) cg_update_row(P, j, full, late && !full, a, rz, rr);
   double t = block_sum(rz, sh);
   if (threadIdx.x == 0) gp(P.part)[lb] = t;
   t = block_sum(rr, sh);
@@ -1156,7 +1203,7 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_dir(const Prob *__restrict__ prob
     if (stop) sc[i_act] = 0.0;
   }
   if (stop || j >= P.M) return;
-  cg_dir_row(P, j, full, vrz / sc[i_b1]);
+  cg_dir_row(P, j, full, late, vrz / sc[i_b1]);
 }
 
 // Start of PCG (pqdts.f90:2024-2031), one thread block per problem: sums of

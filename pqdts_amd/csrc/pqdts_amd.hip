@@ -156,6 +156,7 @@ struct pqdts_ctx {
   int gram_nch = 0;          // most band chunks of one G tile
   double *hw = nullptr;      // hs x M x N partial products when hs > 1
   float *p32 = nullptr;      // FP32 expanded CG direction (mixed precision; Prob::p32)
+  double *pb = nullptr;      // Prob::pb (mixed precision)
   bool use_gram() const { return hvp_pref == 2 || (hvp_pref == 0 && gram_auto); }
   std::unique_ptr<Engine> eng;  // standalone engine (this problem alone, on stream s)
   Engine &engine();
@@ -293,6 +294,7 @@ struct Engine {
     d.cglog = dlog + (int64_t)slot * CG_LOG_ROUNDS * 5;
     // the CG kernels write the FP32 direction copy only for mixed-precision slots
     d.p32 = fp32[slot] ? c->p32 : nullptr;
+    d.pb = c->pb;
     if (!private_scal) {
       if (!dscal) dscal = dalloc<double>((int64_t)NSCAL * cap);
       d.scal = dscal + (int64_t)NSCAL * slot;
@@ -449,7 +451,7 @@ struct Engine {
         wl.pid[i] = p;
         wl.boff[i] = tot;
         tot += nb(p);
-        wl.flags[i] = ctx[p]->full ? F_FULL : 0;
+        wl.flags[i] = (ctx[p]->full ? F_FULL : 0) | (late(p) ? F_LATE : 0);
         wl.a[i] = arg[p];
       }
       wl.boff[wl.np] = tot;
@@ -472,6 +474,20 @@ struct Engine {
     const pqdts_ctx *c = ctx[p];
     const int nt = band == BAND_F ? c->bf.ntiles : band == BAND_FT ? c->bt.ntiles : c->bg.ntiles;
     return cdiv(c->N, TILE) * nt;
+  }
+
+  // Inside the CG of a mixed-precision solve (cg_late), the fold of the Hessian
+  // product moves from the product's epilogue to kb_cg_update (F_LATE), except
+  // for an explicitly split Gram product, whose summing pass folds.
+  bool cg_late = false;
+  bool late(int p) const {
+    return cg_late && fp32[p] && !(ctx[p]->use_gram() && ctx[p]->hs > 1 && ctx[p]->hs_pref > 0);
+  }
+  std::vector<int> early_of(const std::vector<int> &S) const {
+    std::vector<int> r;
+    for (int p : S)
+      if (!late(p)) r.push_back(p);
+    return r;
   }
 
   std::vector<int> reduced_of(const std::vector<int> &S) const {
@@ -564,7 +580,7 @@ struct Engine {
     split_off(g32, g32s);
     std::vector<int> G = g64;
     G.insert(G.end(), g32.begin(), g32.end());
-    std::vector<int> GR = reduced_of(G);
+    std::vector<int> GR = reduced_of(early_of(G));
     if (!GR.empty()) {
       hipEvent_t e0 = kbegin();
       launch(GR, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
@@ -608,7 +624,7 @@ struct Engine {
         gemm_t<float, B_EXPAND, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F32");
       else
         gemm_t<double, B_EXPAND, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F");
-      std::vector<int> FR = reduced_of(FS);
+      std::vector<int> FR = reduced_of(early_of(FS));
       if (!FR.empty())
         launch(FR, [&](int p) { return nb_rows(p); },
                [&](const WorkList &wl, int tot) { kb_foldval<T_HVP><<<tot, BLOCK, 0, s>>>(dprobs, wl, dd); });
@@ -945,6 +961,7 @@ void Engine::tick() {
   if (!A1.empty()) transform(A1, false);  // pqdts.f90:1940-1944
   dL(A, X, G);
   if (!A2.empty()) {
+    cg_late = true;
     cg_init(A2);  // pqdts.f90:1986-2031
     // PCG, pqdts.f90:2036-2090. The exit test runs on the device: every CG launch
     // skips problems whose flag dropped, and the host only checks every few rounds.
@@ -969,6 +986,7 @@ void Engine::tick() {
       guard = -1;
       kround = 0;
     }
+    cg_late = false;
     fetch();
     kresolve();
     HIPC(hipMemcpyAsync(hlog, dlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipMemcpyDeviceToHost, s));
@@ -1108,6 +1126,7 @@ void ensure_fp32(pqdts_ctx *c) {
     HIPC(hipGetLastError());
   }
   if (!c->p32) c->p32 = dalloc<float>(c->MN);
+  if (!c->pb) c->pb = dalloc<double>(c->M);
   HIPC(hipStreamSynchronize(c->s));
   refresh_desc(c);
   if (c->eng) c->eng->attach(0, c);
@@ -1214,7 +1233,7 @@ void destroy_ctx(pqdts_ctx *c) {
   free_band(c->bg);
   void *ptrs[] = {c->flo,       c->flen,     c->foff, c->fval, c->fhas, c->kd_lo, c->kd_hi, c->P,
                   c->O,         c->diag_full, c->diag_red, c->maxr, c->ik,  c->ev,    c->gm,    c->lam,
-                  c->lmin,      c->part,     c->scal, c->ddesc, c->pn, c->hw, c->p32};
+                  c->lmin,      c->part,     c->scal, c->ddesc, c->pn, c->hw, c->p32, c->pb};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (double *p : c->slot)
