@@ -1031,23 +1031,22 @@ __global__ __launch_bounds__(BLOCK) void kb_axpby(const Prob *__restrict__ probs
 // Active set + CG start (pqdts.f90:1986-2031): ik = X < tol and G > 0, G[ik] = 0;
 // partials of nact (region 0), |G|^2 (region 1) and r.z (region 2); r = -G;
 // z = r / sqrt(diag); p = z; sn = 0; ev[j] = -(sum p[j,:] - p[j,maxr]) when reduced.
-// FP32 copy of row j of the CG direction (operand 3) with its implicit column
-// (mr, -1 in full space) replaced by evj: the B operand of the mixed-precision
-// Hessian product, rounded once here instead of in every tile that reads it.
-// Each lane re-reads the entries it has just written.
-// With `late` (F_LATE, reduced problems) also pb[j] = sum of p[j,n] over the free
-// entries (ik = 0) outside the implicit column, for the Hessian product's dot.
-__device__ __forceinline__ void store_p32_row(const Prob &P, int j, int mr, double evj, bool late) {
-  const int64_t base = (int64_t)j * P.N;
-  const GLOBAL double *p = gp(P.op[3]) + base;
-  const GLOBAL uint8_t *ik = gp(P.ik) + base;
-  GLOBAL float *d = gp(P.p32) + base;
-  double b = 0.0;
-  for (int n = lane_id(); n < P.N; n += WAVE) {
-    const double v = p[n];
-    d[n] = (float)(n == mr ? evj : v);
-    if (late && n != mr && !ik[n]) b += v;
+// FP32 copy of the CG direction (operand 3) with the implicit column (mr, -1 in
+// full space) replaced by ev[j]: the B operand of the mixed-precision Hessian
+// product, rounded once where the direction is formed instead of in every tile
+// that reads it. p32_elem runs for each entry v = p[j,n] in the loop that forms
+// the row; with `late` (F_LATE, reduced problems) it also sums b = p over the
+// free entries (ik = 0) outside the implicit column, which p32_row_end stores as
+// pb[j] for the Hessian product's dot, with the implicit column's entry.
+__device__ __forceinline__ void p32_elem(const Prob &P, int64_t idx, int n, int mr, bool late, bool active, double v,
+                                         double &b) {
+  if (n != mr) {
+    gp(P.p32)[idx] = (float)v;
+    if (late && !active) b += v;
   }
+}
+__device__ __forceinline__ void p32_row_end(const Prob &P, int j, int mr, bool late, double evj, double b) {
+  if (mr >= 0 && lane_id() == 0) gp(P.p32)[(int64_t)j * P.N + mr] = (float)evj;
   if (late) {
     b = wave_sum(b);
     if (lane_id() == 0) gp(P.pb)[j] = b;
@@ -1061,7 +1060,8 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
   if (j < P.M) {
     const double sq = sqrt(full ? gp(P.diag_full)[j] : gp(P.diag_red)[j]);
     const int mr = full ? -1 : gp(P.maxr)[j];
-    double psum = 0.0, own = 0.0;
+    double psum = 0.0, own = 0.0, pb = 0.0;
+    const bool lt = late && !full;
     const int64_t base = (int64_t)j * N;
     for (int n = lane_id(); n < N; n += WAVE) {
       const int64_t idx = base + n;
@@ -1077,12 +1077,12 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
       const double r = g * -1.0;
       const double zz = r / sq;
       gp(P.op[1])[idx] = r;
-      gp(P.op[5])[idx] = zz;
       gp(P.op[3])[idx] = zz;
       gp(P.op[4])[idx] = 0.0;
       b1 = fma(r, zz, b1);
       psum += zz;
       if (n == mr) own = zz;
+      if (P.p32) p32_elem(P, idx, n, mr, lt, a != 0, zz, pb);
     }
     double evj = 0.0;
     if (!full) {
@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
       evj = -(psum - own);
       if (lane_id() == 0) gp(P.ev)[j] = evj;
     }
-    if (P.p32) store_p32_row(P, j, mr, evj, late && !full);
+    if (P.p32) p32_row_end(P, j, mr, lt, evj, pb);
   }
   double t = block_sum(nact, sh);
   if (threadIdx.x == 0) gp(P.part)[lb] = t;
@@ -1103,7 +1103,8 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
 }
 
 // One row j of the CG update, one wave: sn += a p; r += (-a) Hp; z = r / sqrt(diag);
-// r.z and r.r accumulate into rz, rr (pqdts.f90:2060-2070).
+// r.z and r.r accumulate into rz, rr (pqdts.f90:2060-2070). z is not stored: the
+// direction update forms the same quotient from r.
 // With `lf` (F_LATE in reduced space) Hp is folded here: 0 at the implicit column
 // and on the active set, Hp - gm[j] elsewhere.
 __device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, bool lf, double a, double &rz,
@@ -1121,25 +1122,28 @@ __device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, b
     const double r = gp(P.op[1])[idx] + (-a) * hp;
     gp(P.op[1])[idx] = r;
     const double zz = r / sq;
-    gp(P.op[5])[idx] = zz;
     rz = fma(r, zz, rz);
     rr = fma(r, r, rr);
   }
 }
 
-// One row j of the new direction, one wave: p = beta p + 1 z (pqdts.f90:2075) and,
-// in reduced space, ev[j] = -(sum p[j,:] - p[j,maxr]) for the next product.
+// One row j of the new direction, one wave: p = beta p + 1 z with z = r / sqrt(diag)
+// (pqdts.f90:2075) and, in reduced space, ev[j] = -(sum p[j,:] - p[j,maxr]) for
+// the next product.
 __device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, bool late, double beta) {
   const int N = P.N;
   const int mr = full ? -1 : gp(P.maxr)[j];
-  double psum = 0.0, own = 0.0;
+  const double sq = sqrt(full ? gp(P.diag_full)[j] : gp(P.diag_red)[j]);
+  const bool lt = late && !full;
+  double psum = 0.0, own = 0.0, pb = 0.0;
   const int64_t base = (int64_t)j * N;
   for (int n = lane_id(); n < N; n += WAVE) {
     const int64_t idx = base + n;
-    const double p = beta * gp(P.op[3])[idx] + 1.0 * gp(P.op[5])[idx];
+    const double p = beta * gp(P.op[3])[idx] + 1.0 * (gp(P.op[1])[idx] / sq);
     gp(P.op[3])[idx] = p;
     psum += p;
     if (n == mr) own = p;
+    if (P.p32) p32_elem(P, idx, n, mr, lt, lt && gp(P.ik)[idx] != 0, p, pb);
   }
   double evj = 0.0;
   if (!full) {
@@ -1148,7 +1152,7 @@ __device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, bool
     evj = -(psum - own);
     if (lane_id() == 0) gp(P.ev)[j] = evj;
   }
-  if (P.p32) store_p32_row(P, j, mr, evj, late && !full);
+  if (P.p32) p32_row_end(P, j, mr, lt, evj, pb);
 }
 
 // CG update (pqdts.f90:2060-2070). Every block first sums the Hessian product's

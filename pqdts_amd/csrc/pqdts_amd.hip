@@ -724,8 +724,8 @@ struct Engine {
       kb_cg_init<<<tot, BLOCK, 0, s>>>(dprobs, wl, 1);
       kb_cg_start<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, S_NACT, S_ND, S_B1, S_EPSCG, S_CGACT, S_CGN);
     });
-    // reads X, G; writes the mask, r, z, p, sn
-    kend(e0, "cg_init", S, [&](int p) { return stream_work(p, 49.0); });
+    // reads X, G; writes the mask, r, p, sn
+    kend(e0, "cg_init", S, [&](int p) { return stream_work(p, 41.0); });
   }
   // CG round j: update (after H p with its v2 partials), then the direction,
   // the round's log and the exit test
@@ -734,15 +734,15 @@ struct Engine {
     launch(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
       kb_cg_update<<<tot, BLOCK, 0, s>>>(dprobs, wl, i_b1, R_B, S_V2);
     });
-    // reads sn, p, r, Hp; writes sn, r, z
-    kend(e0, "cg_update", S, [&](int p) { return stream_work(p, 56.0); });
+    // reads sn, p, r, Hp; writes sn, r
+    kend(e0, "cg_update", S, [&](int p) { return stream_work(p, 48.0); });
   }
   void cg_dir(const std::vector<int> &S, int i_b1, int i_rz, int j) {
     hipEvent_t e0 = kbegin();
     launch(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
       kb_cg_dir<<<tot, BLOCK, 0, s>>>(dprobs, wl, i_b1, i_rz, S_RR, j, S_EPSCG, S_CGACT, S_CGN);
     });
-    // reads p, z; writes p
+    // reads p, r; writes p
     kend(e0, "cg_dir", S, [&](int p) { return stream_work(p, 24.0); }, true);
   }
   void twometric(const std::vector<int> &S) {
