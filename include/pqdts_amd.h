@@ -185,7 +185,7 @@ int pqdts_set_hvp_path(pqdts_ctx *ctx, int path);
  * shape (the default; FP64 products only), 1 = no split, 2..16 fixed (FP64 and
  * FP32 products). Same operator, different
  * rounding; a problem's split never depends on what it is batched with. Call
- * before pqdts_batch_add or between solves. No counterpart in the reference. */
+ * before pqdts_batch_admit or between solves. No counterpart in the reference. */
 int pqdts_set_hvp_split(pqdts_ctx *ctx, int split);
 
 /* ---- batched / served form (no counterpart in the reference) -------------
