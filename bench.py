@@ -193,9 +193,9 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing,
     results, kernel stats). With lanes > 1 the problems are split by size (M x N, equal counts) into
     that many batches, each on its own stream and driven by its own host thread, so that small
     problems advance at their own pace instead of at the pace of the largest ones. lane_cus (one
-    count per lane, smallest problems first) gives every lane compute units of its own: the
-    small lanes' short, latency-bound launches then never wait for CUs held by the long
-    workgroups of the large problems' products."""
+    entry per lane, smallest problems first: a range of CU indices, or None for all) restricts
+    where each lane's workgroups run, so that the small problems' short, latency-bound launches
+    find CUs that the large problems' long workgroups do not hold."""
     import threading
     for p, c in zip(problems, ctxs):
         c.pset(1.0 / p.N, "X")
@@ -211,9 +211,8 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, capacity, timing,
         try:
             cap = min(capacity, len(idx)) if capacity else len(idx)
             b = pqdts_amd.Batch(max(cap, 1))
-            if lane_cus and k > 1:
-                first = sum(lane_cus[:g])
-                b.set_cus(range(first, first + lane_cus[g]))
+            if lane_cus and k > 1 and lane_cus[g] is not None:
+                b.set_cus(lane_cus[g])
             if timing:
                 b.kernel_timing(True)
             pending, slot_of, steps = list(idx), {}, 0
@@ -271,12 +270,46 @@ def calibration_copy(pqdts_amd, problems, ctxs):
 
 
 def parse_lane_cus(args, pqdts_amd):
-    if not args.lane_cus:
+    """--lane-cus: "none" (every lane on every CU); "auto" (3 lanes: the smallest problems on
+    the first eighth of the CUs alone, the middle lane on all, the largest problems on all but
+    the first quarter); counts "a,b,c" (consecutive exclusive partitions); or one consecutive
+    range per lane, "lo-hi" or "all" (e.g. 0-31,all,64-255). Returns one range (or None for
+    all CUs) per lane."""
+    spec = args.lane_cus.strip()
+    ncu = pqdts_amd.device_cu_count()
+    if spec in ("", "none"):
         return None
-    cus = [int(x) for x in args.lane_cus.split(",")]
-    if len(cus) != args.lanes or sum(cus) > pqdts_amd.device_cu_count():
-        raise SystemExit(f"--lane-cus needs {args.lanes} counts summing to at most the device's CUs")
-    return cus
+    if spec == "auto":
+        if args.lanes != 3 or ncu < 16:
+            return None
+        return [range(0, ncu // 8), None, range(ncu // 4, ncu)]
+    items = spec.split(",")
+    if len(items) != args.lanes:
+        raise SystemExit(f"--lane-cus needs one entry per lane ({args.lanes})")
+    if all(x.isdigit() for x in items):
+        cus = [int(x) for x in items]
+        if sum(cus) > ncu or min(cus) < 1:
+            raise SystemExit("--lane-cus counts must be positive and sum to at most the device's CUs")
+        return [range(sum(cus[:g]), sum(cus[:g + 1])) for g in range(len(cus))]
+    out = []
+    for x in items:
+        if x == "all":
+            out.append(None)
+            continue
+        parts = x.split("-")
+        if len(parts) != 2 or not all(v.isdigit() for v in parts):
+            raise SystemExit(f"--lane-cus entry {x}: expected lo-hi or all")
+        lo, hi = int(parts[0]), int(parts[1])
+        if not 0 <= lo <= hi < ncu:
+            raise SystemExit(f"--lane-cus range {x} outside 0-{ncu - 1}")
+        out.append(range(lo, hi + 1))
+    return out
+
+
+def lane_cus_str(lane_cus):
+    if lane_cus is None:
+        return None
+    return ["all" if r is None else f"{r.start}-{r.stop - 1}" for r in lane_cus]
 
 
 def workload_mix(args, pqdts_amd, workloads):
@@ -358,7 +391,7 @@ def workload_mix(args, pqdts_amd, workloads):
                                f"1e6-1e8), eps={args.eps:g}, mode 2, from Pi=1/N, batched engine in {args.lanes} "
                                f"size lanes (one stream and host thread each), mixed precision",
                    "count": len(problems), "eps": args.eps, "maxiter_per_stage": args.maxiter, "capacity": cap,
-                   "lanes": args.lanes, "lane_cus": lane_cus},
+                   "lanes": args.lanes, "lane_cus": lane_cus_str(lane_cus)},
         "converged": {"batched_mixed": nconv_m, "of": len(problems) * args.steps, "batched_fp64": nconv_b64,
                       "plain_fp64_sample": int(sum(conv64))},
         "latency_s": {"p50": pct(lat_m, 50), "p99": pct(lat_m, 99), "max": max(lat_m),
@@ -668,8 +701,9 @@ def main():
     ap.add_argument("--maxiter", type=int, default=5000)
     ap.add_argument("--capacity", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=3)
-    ap.add_argument("--lane-cus", default="",
-                    help="compute units per lane, smallest problems first (e.g. 48,80,128); empty: shared")
+    ap.add_argument("--lane-cus", default="auto",
+                    help="CUs per lane, smallest problems first: auto, none, counts (48,80,128) or ranges "
+                         "(0-31,all,64-255)")
     ap.add_argument("--seq-stride", type=int, default=8)
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--hbm-gb", type=float, default=200.0)
