@@ -5,6 +5,8 @@
                                                           (+ VGPRs, LDS bytes, grid)
   python -m pqdts_amd.profsum last <db> <kernel> <k>      mean counter values of the last k dispatches
                                                           of the kernel whose name starts with <kernel>
+  python -m pqdts_amd.profsum streams <db> <kernel>       dispatches of the kernels whose name starts
+                                                          with <kernel>, per stream: calls / avg / total
 Kernel names are shortened to their template head.
 """
 import sqlite3
@@ -65,8 +67,24 @@ def last(db: str, kernel: str, k: int):
               f"{sum(vals) / len(vals):.1f} per dispatch, values {', '.join(f'{v:.0f}' for v in vals)}")
 
 
+def streams(db: str, kernel: str):
+    c = sqlite3.connect(db)
+    cols = [r[1] for r in c.execute("pragma table_info(kernels)").fetchall()]
+    key = next((k for k in ("stream_id", "queue_id", "stream", "queue") if k in cols), None)
+    if key is None:
+        raise SystemExit(f"no stream or queue column in kernels: {cols}")
+    rows = c.execute(f"select name, {key}, count(*), avg(duration), sum(duration) from kernels "
+                     f"group by name, {key} order by name, min(start)").fetchall()
+    print(f"{'kernel':58s} {key:>10s} {'calls':>7s} {'avg_us':>10s} {'total_ms':>10s}")
+    for name, sid, calls, avg, tot in rows:
+        if short(name).startswith(kernel):
+            print(f"{short(name)[:58]:58s} {str(sid):>10s} {calls:7d} {avg / 1e3:10.2f} {tot / 1e6:10.2f}")
+
+
 if __name__ == "__main__":
-    if sys.argv[1] == "trace":
+    if sys.argv[1] == "streams":
+        streams(sys.argv[2], sys.argv[3])
+    elif sys.argv[1] == "trace":
         trace(sys.argv[2])
     elif sys.argv[1] == "last":
         last(sys.argv[2], sys.argv[3], int(sys.argv[4]))
