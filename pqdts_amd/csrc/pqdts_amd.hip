@@ -19,6 +19,7 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -226,6 +227,14 @@ struct Engine {
   double *dscal = nullptr;        // device scalars [cap * NSCAL] (batched engines)
   double *hscal = nullptr;        // pinned host copy
   bool private_scal = false;      // standalone: scalars live in the context
+  // Captured CG blocks (hipGraph): the launches of CG rounds j0..j1 for one set of
+  // problems, replayed while that set, its spaces and precisions stay the same
+  // (the tail of a batch repeats one set for thousands of Newton iterations).
+  // Batched engines only, without kernel timing or synchronising timers.
+  std::map<std::vector<int64_t>, hipGraphExec_t> cg_graphs;
+  bool graphs_on = false;
+  std::vector<int64_t> gen;  // per slot: attach count, so a graph never outlives its problem
+  int64_t gen_next = 0;
   std::vector<double> arg;        // per-slot scalar argument of the next launch
   int guard = -1;                 // launches skip problems whose scal[guard] is 0
   double *dlog = nullptr;         // device-side CG logs [cap][CG_LOG_ROUNDS][5]
@@ -284,12 +293,54 @@ struct Engine {
     dlog = dalloc<double>((int64_t)cap * CG_LOG_ROUNDS * 5);
     HIPC(hipHostMalloc((void **)&hlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipHostMallocDefault));
     sync_timers = getenv("PQDTS_SYNC_TIMERS") && atoi(getenv("PQDTS_SYNC_TIMERS")) != 0;
+    graphs_on = getenv("PQDTS_GRAPHS") && atoi(getenv("PQDTS_GRAPHS")) != 0;
+  }
+  void drop_graphs() {
+    for (auto &kv : cg_graphs) (void)hipGraphExecDestroy(kv.second);
+    cg_graphs.clear();
+  }
+  // Runs CG rounds j0..j1 of the problems S through a captured graph: captured
+  // on first use for this key (rounds, slots, their contexts, spaces,
+  // precisions, Gram splits), replayed afterwards. The launches are the eager
+  // ones, so the results are the same bits.
+  template <class R>
+  void graph_rounds(const std::vector<int> &S, int j0, int j1, R &&rounds) {
+    std::vector<int64_t> key{j0, j1, (int64_t)S.size()};
+    for (int p : S) {
+      const pqdts_ctx *c = ctx[p];
+      key.push_back(p);
+      key.push_back(gen[p]);
+      key.push_back((c->full ? 1 : 0) | (late(p) ? 2 : 0) | (fp32[p] ? 4 : 0) | (c->use_gram() ? 8 : 0) |
+                    ((int64_t)c->hs << 4) | ((int64_t)c->hs_pref << 12));
+    }
+    auto it = cg_graphs.find(key);
+    if (it == cg_graphs.end()) {
+      if (cg_graphs.size() >= 256) drop_graphs();
+      hipGraph_t g = nullptr;
+      HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        rounds(j0, j1);
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPC(hipStreamEndCapture(s, &g));
+      hipGraphExec_t ex = nullptr;
+      const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPC(e);
+      it = cg_graphs.emplace(std::move(key), ex).first;
+    }
+    HIPC(hipGraphLaunch(it->second, s));
   }
 
   // Places a context in a slot; a batched engine redirects its scalars into the
   // engine's contiguous array so that one copy fetches every problem's scalars.
   void attach(int slot, pqdts_ctx *c) {
     ctx[slot] = c;
+    if ((int)gen.size() < cap) gen.assign(cap, -1);
+    gen[slot] = ++gen_next;
     Prob d = c->desc;
     d.cglog = dlog + (int64_t)slot * CG_LOG_ROUNDS * 5;
     // the CG kernels write the FP32 direction copy only for mixed-precision slots
@@ -787,6 +838,7 @@ struct PState {
 
 Engine::~Engine() {
   if (s) (void)hipStreamSynchronize(s);
+  drop_graphs();
   for (auto &p : kpending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -968,20 +1020,28 @@ void Engine::tick() {
     {
       Scope sc(this, "CG");
       guard = S_CGACT;
-      int ib1 = S_B1;
-      for (int j = 1; j <= 50; j++) {
-        kround = j;
-        BF(A2, XT, HP, true, true, true, true);
-        const int i_rz = (j & 1) ? S_RZ1 : S_RZ0;
-        cg_update(A2, ib1);
-        cg_dir(A2, ib1, i_rz, j);
-        ib1 = i_rz;
-        if (j % 8 == 0 || j == 50) {
-          fetch();
-          bool any = false;
-          for (int p : A2) any = any || host_scal(p)[S_CGACT] != 0.0;
-          if (!any) break;
+      // round j: r.z of the previous round is in S_B1 (j = 1), else S_RZ0 / S_RZ1
+      auto ib1_of = [](int j) { return j == 1 ? (int)S_B1 : ((j - 1) & 1) ? (int)S_RZ1 : (int)S_RZ0; };
+      auto rounds = [&](int j0, int j1) {
+        for (int j = j0; j <= j1; j++) {
+          kround = j;
+          const int ib1 = ib1_of(j), i_rz = (j & 1) ? S_RZ1 : S_RZ0;
+          BF(A2, XT, HP, true, true, true, true);
+          cg_update(A2, ib1);
+          cg_dir(A2, ib1, i_rz, j);
         }
+      };
+      const bool graph = graphs_on && !private_scal && !ktiming && !sync_timers;
+      for (int j0 = 1; j0 <= 50; j0 += 8) {
+        const int j1 = std::min(j0 + 7, 50);
+        if (graph)
+          graph_rounds(A2, j0, j1, rounds);
+        else
+          rounds(j0, j1);
+        fetch();
+        bool any = false;
+        for (int p : A2) any = any || host_scal(p)[S_CGACT] != 0.0;
+        if (!any) break;
       }
       guard = -1;
       kround = 0;

@@ -113,6 +113,32 @@ def test_launch_forms_same_bits(precision):
     assert np.array_equal(xa, xb), f"max diff {np.max(np.abs(xa - xb)):.3e}"
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_replay_same_bits(monkeypatch, precision):
+    """CG rounds replayed from captured hipGraphs (PQDTS_GRAPHS=1) give the bits of
+    the eager launches, over whole solves: graphs are replayed while the set of
+    problems in CG stays the same and captured anew when it changes."""
+    probs = problems()
+
+    def run(graphs):
+        monkeypatch.setenv("PQDTS_GRAPHS", "1" if graphs else "0")
+        b = pqdts_amd.Batch(8)
+        ctxs = []
+        for p in probs:
+            c = pqdts_amd.Context.from_problem(p)
+            start(c, p)
+            ctxs.append(c)
+            b.admit(c, 2, 300, p.eps, 1, 0, precision)
+        res = b.run()
+        return [c.get("X") for c in ctxs], [res[i] for i in range(len(probs))]
+
+    xa, ra = run(False)
+    xb, rb = run(True)
+    for p, a, b_ in zip(probs, xa, xb):
+        assert np.array_equal(a, b_), f"problem {p.name}"
+    assert ra == rb
+
+
 def test_continuous_admission_matches_alone():
     probs = problems()
     alone = [solve_alone(p) for p in probs]
