@@ -184,8 +184,8 @@ int pqdts_set_hvp_path(pqdts_ctx *ctx, int path);
  * dependency chain of products too small to fill the GPU. 0 = by the problem's
  * shape (the default; FP64 products only), 1 = no split, 2..16 fixed (FP64 and
  * FP32 products). Same operator, different
- * rounding; a problem's split never depends on what it is batched with. Call
- * before pqdts_batch_admit or between solves. No counterpart in the reference. */
+ * rounding; a problem's split never depends on what it is batched with. Fails
+ * while the context is in a batch. No counterpart in the reference. */
 int pqdts_set_hvp_split(pqdts_ctx *ctx, int split);
 
 /* ---- batched / served form (no counterpart in the reference) -------------
@@ -195,8 +195,10 @@ int pqdts_set_hvp_split(pqdts_ctx *ctx, int split);
  * problem follows exactly the reference's algorithm on its own data (its own
  * stages, CG exits and line searches); a problem's result does not depend on
  * what else is in the batch. Contexts stay owned by the caller and must not be
- * used through the single-context calls while admitted; call
- * pqdts_set_hvp_path before admitting. */
+ * used through the single-context calls while admitted. A context is in at most
+ * one slot at a time (admitting it again fails until pqdts_batch_release);
+ * pqdts_set_hvp_path and pqdts_set_hvp_split fail while it is admitted;
+ * pqdts_destroy of an admitted context first releases its slot. */
 typedef struct pqdts_batch pqdts_batch;
 int pqdts_batch_create(int capacity, pqdts_batch **out);
 int pqdts_batch_destroy(pqdts_batch *b);

@@ -160,6 +160,7 @@ struct pqdts_ctx {
   double *pb = nullptr;      // Prob::pb (mixed precision)
   bool use_gram() const { return hvp_pref == 2 || (hvp_pref == 0 && gram_auto); }
   std::unique_ptr<Engine> eng;  // standalone engine (this problem alone, on stream s)
+  Engine *batch = nullptr;      // the batch engine whose slot holds this context, if any
   Engine &engine();
 };
 
@@ -354,6 +355,7 @@ struct Engine {
     HIPC(hipStreamSynchronize(s));
   }
   void detach(int slot) {
+    if (ctx[slot] && ctx[slot]->batch == this) ctx[slot]->batch = nullptr;
     ctx[slot] = nullptr;
     st[slot].reset();
   }
@@ -1286,6 +1288,14 @@ void apply_gram_split(pqdts_ctx *c) {
 
 void destroy_ctx(pqdts_ctx *c) {
   if (!c) return;
+  if (c->batch) {
+    // still in a batch slot: leave it first, so the batch never sees a freed context
+    Engine *e = c->batch;
+    if (e->s) (void)hipStreamSynchronize(e->s);
+    for (int p = 0; p < e->cap; p++)
+      if (e->ctx[p] == c) e->detach(p);
+    c->batch = nullptr;
+  }
   if (c->s) (void)hipStreamSynchronize(c->s);
   c->eng.reset();
   free_band(c->bf);
@@ -1883,6 +1893,7 @@ int pqdts_set_hvp_path(pqdts_ctx *ctx, int path) {
   return guarded([&] {
     checked(ctx);
     require(path >= 0 && path <= 2, "hvp path must be 0 (auto), 1 (F products) or 2 (Gram)");
+    require(ctx->batch == nullptr, "the context is in a batch (release it first)");
     ctx->hvp_pref = path;
     if (ctx->use_gram()) ensure_gram(ctx);
   });
@@ -1892,6 +1903,7 @@ int pqdts_set_hvp_split(pqdts_ctx *ctx, int split) {
   return guarded([&] {
     checked(ctx);
     require(split >= 0 && split <= 16, "split must be 0 (by shape) or 1..16");
+    require(ctx->batch == nullptr, "the context is in a batch (release it first)");
     ctx->hs_pref = split;
     apply_gram_split(ctx);
   });
@@ -1955,7 +1967,12 @@ int pqdts_batch_create(int capacity, pqdts_batch **out) {
 }
 
 int pqdts_batch_destroy(pqdts_batch *b) {
-  return guarded([&] { delete b; });
+  return guarded([&] {
+    if (b)
+      for (int p = 0; p < b->eng.cap; p++)
+        if (b->eng.ctx[p] && b->eng.ctx[p]->batch == &b->eng) b->eng.ctx[p]->batch = nullptr;
+    delete b;
+  });
 }
 
 int pqdts_batch_admit(pqdts_batch *b, pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage,
@@ -1966,6 +1983,7 @@ int pqdts_batch_admit(pqdts_batch *b, pqdts_ctx *ctx, int mode, int maxiter, dou
     require(mode == 1 || mode == 2, "mode must be 1 or 2");
     require(start_stage == 1 || start_stage == 2, "start_stage must be 1 or 2");
     require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
+    require(ctx->batch == nullptr, "the context is already in a batch (release it first)");
     Engine &e = b->eng;
     int p = -1;
     for (int i = 0; i < e.cap; i++)
@@ -1978,6 +1996,7 @@ int pqdts_batch_admit(pqdts_batch *b, pqdts_ctx *ctx, int mode, int maxiter, dou
     if (precision) ensure_fp32(ctx);
     e.fp32[p] = precision;
     e.attach(p, ctx);
+    ctx->batch = &e;
     PState *q = new PState();
     q->mode = mode;
     q->maxiter = maxiter;

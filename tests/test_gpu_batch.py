@@ -139,6 +139,36 @@ def test_graph_replay_same_bits(monkeypatch, precision):
     assert ra == rb
 
 
+def test_batch_membership_rules():
+    """A context sits in at most one batch slot; while it does, the Hessian-path
+    settings (which rebuild its device descriptor) are refused; release frees it;
+    destroying a context before its batch leaves the batch usable."""
+    p = problems()[1]
+    c = pqdts_amd.Context.from_problem(p)
+    start(c, p)
+    b1, b2 = pqdts_amd.Batch(2), pqdts_amd.Batch(2)
+    s1 = b1.admit(c, 2, 5, p.eps, 1, 0, 0)
+    with pytest.raises(pqdts_amd.PqdtsError):
+        b2.admit(c, 2, 5, p.eps, 1, 0, 0)
+    with pytest.raises(pqdts_amd.PqdtsError):
+        c.set_hvp_split(2)
+    with pytest.raises(pqdts_amd.PqdtsError):
+        c.set_hvp_path("F")
+    b1.release(s1)
+    c.set_hvp_path("G")
+    s2 = b2.admit(c, 2, 5, p.eps, 1, 0, 0)
+    b2.run()
+    b2.release(s2)
+    # destroyed while still admitted: the batch drops the slot, then closes cleanly
+    d = pqdts_amd.Context.from_problem(p)
+    start(d, p)
+    b1.admit(d, 2, 5, p.eps, 1, 0, 0)
+    d.close()
+    assert pqdts_amd._lib.load().pqdts_batch_active(b1.h) == 0
+    b1.close()
+    b2.close()
+
+
 def test_continuous_admission_matches_alone():
     probs = problems()
     alone = [solve_alone(p) for p in probs]
