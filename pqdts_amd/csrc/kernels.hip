@@ -110,11 +110,11 @@ __device__ __forceinline__ int64_t lmin64(int64_t a, int64_t b) { return a < b ?
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 __device__ __forceinline__ int wave_id() { return threadIdx.x / WAVE; }
 
-// problem slot i of the work list that owns flat block `bid`
-// Entry of block bid: the last i with boff[i] <= bid (boff is nondecreasing from
-// boff[0] = 0), i.e. the number of such entries minus one. Every lane tests
-// MAXB / WAVE entries and a ballot counts them: one round of independent loads
-// instead of a binary search's chain of dependent ones. Wave-uniform result.
+// Work-list entry of flat block bid: the last i with boff[i] <= bid (boff is
+// nondecreasing from boff[0] = 0), i.e. the number of such entries minus one.
+// Every lane tests MAXB / WAVE entries and a ballot counts them: one round of
+// independent loads instead of a binary search's chain of dependent ones.
+// Wave-uniform result.
 __device__ __forceinline__ int wl_find(const WorkList &wl, int bid) {
   const int l = lane_id();
   int cnt = 0;
@@ -1028,9 +1028,6 @@ __global__ __launch_bounds__(BLOCK) void kb_axpby(const Prob *__restrict__ probs
 // Operands: X=0, G=1 (r), HP=2, XTMP=3 (p), SN=4, Z=5.
 // ---------------------------------------------------------------------------
 
-// Active set + CG start (pqdts.f90:1986-2031): ik = X < tol and G > 0, G[ik] = 0;
-// partials of nact (region 0), |G|^2 (region 1) and r.z (region 2); r = -G;
-// z = r / sqrt(diag); p = z; sn = 0; ev[j] = -(sum p[j,:] - p[j,maxr]) when reduced.
 // FP32 copy of the CG direction (operand 3) with the implicit column (mr, -1 in
 // full space) replaced by ev[j]: the B operand of the mixed-precision Hessian
 // product, rounded once where the direction is formed instead of in every tile
@@ -1053,6 +1050,10 @@ __device__ __forceinline__ void p32_row_end(const Prob &P, int j, int mr, bool l
   }
 }
 
+// Active set + CG start (pqdts.f90:1986-2031): ik = X < tol and G > 0, G[ik] = 0;
+// partials of nact (region 0), |G|^2 (region 1) and r.z (region 2); r = -G;
+// p = z = r / sqrt(diag) (z itself is not stored); sn = 0;
+// ev[j] = -(sum p[j,:] - p[j,maxr]) when reduced; the FP32 copy of p when P.p32.
 __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ probs, WorkList wl, int tactive) {
   __shared__ double sh[WAVES_PER_BLOCK];
   ROW_PROLOGUE();

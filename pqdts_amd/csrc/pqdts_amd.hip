@@ -301,9 +301,9 @@ struct Engine {
     cg_graphs.clear();
   }
   // Runs CG rounds j0..j1 of the problems S through a captured graph: captured
-  // on first use for this key (rounds, slots, their contexts, spaces,
-  // precisions, Gram splits), replayed afterwards. The launches are the eager
-  // ones, so the results are the same bits.
+  // on first use for this key (rounds, slots and their attach generations,
+  // spaces, precisions, Hessian paths and splits), replayed afterwards. The
+  // launches are the eager ones, so the results are the same bits.
   template <class R>
   void graph_rounds(const std::vector<int> &S, int j0, int j1, R &&rounds) {
     std::vector<int64_t> key{j0, j1, (int64_t)S.size()};
@@ -622,7 +622,8 @@ struct Engine {
     }
     // Gram path: fold values, then one product 2 G p~ with the epilogue.
     // Problems with a split product write partials instead; their summing pass
-    // forms the fold values itself.
+    // forms the fold values itself. Problems whose fold is left to kb_cg_update
+    // (late) need no fold values either.
     std::vector<int> g64s, g32s;
     auto split_off = [&](std::vector<int> &v, std::vector<int> &sv) {
       std::vector<int> keep;
