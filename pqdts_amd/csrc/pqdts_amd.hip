@@ -1025,26 +1025,31 @@ void Engine::tick() {
       guard = S_CGACT;
       // round j: r.z of the previous round is in S_B1 (j = 1), else S_RZ0 / S_RZ1
       auto ib1_of = [](int j) { return j == 1 ? (int)S_B1 : ((j - 1) & 1) ? (int)S_RZ1 : (int)S_RZ0; };
+      // problems still in CG; those whose CG stopped leave the launches at the
+      // next host check (their blocks would only exit at the guard)
+      std::vector<int> C = A2;
       auto rounds = [&](int j0, int j1) {
         for (int j = j0; j <= j1; j++) {
           kround = j;
           const int ib1 = ib1_of(j), i_rz = (j & 1) ? S_RZ1 : S_RZ0;
-          BF(A2, XT, HP, true, true, true, true);
-          cg_update(A2, ib1);
-          cg_dir(A2, ib1, i_rz, j);
+          BF(C, XT, HP, true, true, true, true);
+          cg_update(C, ib1);
+          cg_dir(C, ib1, i_rz, j);
         }
       };
       const bool graph = graphs_on && !private_scal && !ktiming && !sync_timers;
       for (int j0 = 1; j0 <= 50; j0 += 8) {
         const int j1 = std::min(j0 + 7, 50);
         if (graph)
-          graph_rounds(A2, j0, j1, rounds);
+          graph_rounds(C, j0, j1, rounds);
         else
           rounds(j0, j1);
         fetch();
-        bool any = false;
-        for (int p : A2) any = any || host_scal(p)[S_CGACT] != 0.0;
-        if (!any) break;
+        std::vector<int> next;
+        for (int p : C)
+          if (host_scal(p)[S_CGACT] != 0.0) next.push_back(p);
+        if (next.empty()) break;
+        C.swap(next);
       }
       guard = -1;
       kround = 0;
