@@ -231,16 +231,17 @@ __global__ __launch_bounds__(BLOCK) void kb_reduce(const Prob *__restrict__ prob
   if (threadIdx.x == 0) gp(P.scal)[dst] = v;
 }
 
-// Sums of up to three regions (scal[d_i] = sum of pn[r_i] partials of region r_i),
-// one block per work-list entry.
+// Sums of up to three regions (scal[d_i] = sum of pn[r_i] partials of region r_i;
+// the minimum instead for the regions whose bit is set in minmask), one block per
+// work-list entry.
 __global__ __launch_bounds__(BLOCK) void kb_reduce_pn(const Prob *__restrict__ probs, WorkList wl, int nr, int r0,
-                                                      int d0, int r1, int d1, int r2, int d2) {
+                                                      int d0, int r1, int d1, int r2, int d2, int minmask) {
   __shared__ double sh[WAVES_PER_BLOCK];
   const Prob &P = probs[wl.pid[blockIdx.x]];
   if (wl.guard >= 0 && gp(P.scal)[wl.guard] == 0.0) return;
   const int rr[3] = {r0, r1, r2}, dd[3] = {d0, d1, d2};
   for (int k = 0; k < nr; k++) {
-    const double v = reduce_parts(gp(P.part) + (int64_t)rr[k] * P.pcap, gp(P.pn)[rr[k]], false, sh);
+    const double v = reduce_parts(gp(P.part) + (int64_t)rr[k] * P.pcap, gp(P.pn)[rr[k]], (minmask >> k) & 1, sh);
     if (threadIdx.x == 0) gp(P.scal)[dd[k]] = v;
   }
 }
@@ -863,6 +864,7 @@ __global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ pro
     double v = sh[0];
     for (int w = 1; w < WAVES_PER_BLOCK; w++) v = fmin(v, sh[w]);
     gp(P.part)[(int64_t)region * P.pcap + lb] = v;
+    if (lb == 0) gp(P.pn)[region] = nblk;
   }
 }
 

@@ -575,7 +575,9 @@ struct Engine {
   }
 
   // L (pqdts.f90:917-973): data term -> scal[ia], smoothing -> scal[ib]
-  void L(const std::vector<int> &S, int x, bool recon, int ia = S_LA, int ib = S_LB) {
+  // ixmin >= 0: the xmin partials kb_project left in region R_C are reduced in the
+  // same launch (a line-search trial)
+  void L(const std::vector<int> &S, int x, bool recon, int ia = S_LA, int ib = S_LB, int ixmin = -1) {
     struct Scope sc(this, "L");
     if (recon) reconstruct(S, x);
     GemmOps ops{BAND_F, x, OP_O, 0, 0, R_A};
@@ -588,7 +590,10 @@ struct Engine {
     launch(S, [&](int p) { return std::max(1, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); },
            [&](const WorkList &wl, int tot) { kb_smooth<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, R_B); });
     launch(S, [](int) { return 1; }, [&](const WorkList &wl, int) {
-      kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 2, R_A, ia, R_B, ib, 0, 0);
+      if (ixmin >= 0)
+        kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 3, R_A, ia, R_B, ib, R_C, ixmin, 4);
+      else
+        kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 2, R_A, ia, R_B, ib, 0, 0, 0);
     });
   }
   double Lval(int p) { return host_scal(p)[S_LA] + host_scal(p)[S_LB]; }
@@ -726,16 +731,21 @@ struct Engine {
   }
 
   // doproj / doproj2 (pqdts.f90:1120-1164) on operand t; with form_trial the
-  // trial point t = arg*SN + X is formed first. xmin -> S_XMIN.
+  // trial point t = arg*SN + X is formed first. xmin -> S_XMIN (ixmin < 0: the
+  // xmin partials stay in region R_C with their count in pn, for L to reduce).
   void project(const std::vector<int> &S, int t, bool form_trial, bool recon_again, int ixmin = S_XMIN) {
     need_maxr(S);
     bool any_full = false, any_red = false;
     for (int p : S) (ctx[p]->full ? any_full : any_red) = true;
     struct Scope sc(this, any_red && !any_full ? "doproj2" : "doproj");
     hipEvent_t e0 = kbegin();
-    launch_reduce(S, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+    auto proj = [&](const WorkList &wl, int tot) {
       kb_project<<<tot, BLOCK, 0, s>>>(dprobs, wl, t, form_trial ? 1 : 0, recon_again ? 1 : 0, R_C);
-    }, R_C, ixmin, true);
+    };
+    if (ixmin >= 0)
+      launch_reduce(S, [&](int p) { return nb_rows(p); }, proj, R_C, ixmin, true);
+    else
+      launch(S, [&](int p) { return nb_rows(p); }, proj);
     kend(e0, "project", S, [&](int p) { return stream_work(p, form_trial ? 24.0 : 16.0); });
   }
 
@@ -879,8 +889,9 @@ void Engine::line_search(const std::vector<int> &A) {
   auto trials = [&](const std::vector<int> &S, int K, const std::function<double(int, int)> &alpha) {
     for (int k = 0; k < K; k++) {
       for (int p : S) arg[p] = alpha(p, k);
-      project(S, PQDTS_XTMP, true, true, S_LTX + k);
-      L(S, PQDTS_XTMP, false, S_LT + k, S_LTB + k);  // the reduced-space prep was applied by kb_project
+      project(S, PQDTS_XTMP, true, true, -1);
+      // the reduced-space prep was applied by kb_project; xmin is reduced with L
+      L(S, PQDTS_XTMP, false, S_LT + k, S_LTB + k, S_LTX + k);
     }
     fetch();
   };
@@ -1925,6 +1936,7 @@ int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n) 
     Prob *pd = nullptr;
     hipError_t e1 = hipMemcpy(d, y, sizeof(double) * rows * n, hipMemcpyHostToDevice);
     double *part = nullptr;
+    int *pn = nullptr;
     if (e1 == hipSuccess) {
       Prob p{};
       p.M = (int)rows;
@@ -1933,6 +1945,9 @@ int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n) 
       p.op[0] = d;
       part = dalloc<double>((int64_t)p.pcap * NREG);
       p.part = part;
+      // kb_project records its partial count
+      pn = dalloc<int>(NREG);
+      p.pn = pn;
       pd = dalloc<Prob>(1);
       e1 = hipMemcpy(pd, &p, sizeof(Prob), hipMemcpyHostToDevice);
       if (e1 == hipSuccess) {
@@ -1952,6 +1967,7 @@ int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n) 
     (void)hipFree(d);
     if (pd) (void)hipFree(pd);
     if (part) (void)hipFree(part);
+    if (pn) (void)hipFree(pn);
     HIPC(e1);
   });
 }
