@@ -34,6 +34,9 @@ constexpr int WAVE = 64;
 constexpr int BLOCK = 256;
 constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
 constexpr int MAXB = 128;      // problems per launch
+// element kernels over the M x N arrays: blocks of BLOCK x ELEMS elements
+constexpr int ELEMS = 8;
+constexpr int EBLOCK = BLOCK * ELEMS;
 constexpr int NSLOT = 7;       // X G HP XTMP SN Z, then O as operand 6
 constexpr int OP_O = 6;
 
@@ -286,6 +289,7 @@ struct GemmOps {
   int e;       // stencil source operand (E_GRAD / E_HVP)
   int dd;      // direction operand for the v2 = dd.C partial
   int region;  // partial region
+  int sm = -1;  // E_LOSS: region of the smoothing partials of B computed by extra blocks (-1: none)
 };
 
 template <typename T>
@@ -579,18 +583,45 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
   }
 }
 
+// Smoothing part of the objective (pqdts.f90:956-962), block lb of nblk: partial
+// of sum gamma (X[j,n] - X[j+1,n])^2 over j < M-1, elements [lb, lb+1) x EBLOCK
+// of the (M-1) x N differences, into part[region][lb]; pn[region] = nblk.
+__device__ __forceinline__ void smooth_part(const Prob &P, int x, int lb, int nblk, int region, double *sh) {
+  const int64_t total = (int64_t)(P.M - 1) * P.N;
+  const GLOBAL double *X = gp(P.op[x]);
+  double s = 0.0;
+  for (int64_t e = (int64_t)lb * EBLOCK + threadIdx.x; e < lmin64(total, (int64_t)(lb + 1) * EBLOCK);
+       e += BLOCK) {
+    const double t = X[e] - X[e + P.N];
+    s += P.gamma * (t * t);
+  }
+  const double b = block_sum(s, sh);
+  if (threadIdx.x == 0) gp(P.part)[(int64_t)region * P.pcap + lb] = b;
+  if (threadIdx.x == 0 && lb == 0) gp(P.pn)[region] = nblk;
+}
+
+// With ops.sm >= 0 (E_LOSS, B = X) the blocks past the product's own compute
+// the smoothing partials of X: one launch for both terms of the objective.
 template <typename T, int KC, int BM, int EM, bool MASK, bool DOT, bool LAT>
 __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ probs, WorkList wl, GemmOps ops) {
   __shared__ T As[2][KC * GT];
   __shared__ T Bs[2][KC * GT];
   __shared__ double sh[WAVES_PER_BLOCK];
   BATCH_PROLOGUE();
+  int ng = nblk;  // blocks of the product itself
+  if (EM == E_LOSS && ops.sm >= 0) {
+    ng = ((P.N + GT - 1) / GT) * P.bf.ntiles;
+    if (lb >= ng) {
+      smooth_part(P, ops.b, lb - ng, nblk - ng, ops.sm, sh);
+      return;
+    }
+  }
   double redv = 0.0;
   band_gemm_tile<T, KC, BM, EM, MASK, DOT, LAT>(P, ops, lb, full, late, As, Bs, redv);
   if (EM == E_LOSS || EM == E_STORE_LOSS || DOT) {
     const double b = block_sum(redv, sh);
     if (threadIdx.x == 0) gp(P.part)[(int64_t)ops.region * P.pcap + lb] = b;
-    if (threadIdx.x == 0 && lb == 0) gp(P.pn)[ops.region] = nblk;
+    if (threadIdx.x == 0 && lb == 0) gp(P.pn)[ops.region] = ng;
   }
 }
 
@@ -900,28 +931,15 @@ __global__ __launch_bounds__(BLOCK) void kb_conv_rows(const Prob *__restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// Element kernels over the M x N arrays: blocks of 256 x ELEMS elements.
+// Element kernels over the M x N arrays: blocks of BLOCK x ELEMS elements.
 // ---------------------------------------------------------------------------
-constexpr int ELEMS = 8;
-constexpr int EBLOCK = BLOCK * ELEMS;
 
-// Smoothing part of the objective (pqdts.f90:956-962): partial of
-// sum gamma (X[j,n] - X[j+1,n])^2 over j < M-1.
+// Smoothing part of the objective alone (smooth_part).
 __global__ __launch_bounds__(BLOCK) void kb_smooth(const Prob *__restrict__ probs, WorkList wl, int x,
                                                    int region) {
   __shared__ double sh[WAVES_PER_BLOCK];
   BATCH_PROLOGUE();
-  const int64_t total = (int64_t)(P.M - 1) * P.N;
-  const GLOBAL double *X = gp(P.op[x]);
-  double s = 0.0;
-  for (int64_t e = (int64_t)lb * EBLOCK + threadIdx.x; e < lmin64(total, (int64_t)(lb + 1) * EBLOCK);
-       e += BLOCK) {
-    const double t = X[e] - X[e + P.N];
-    s += P.gamma * (t * t);
-  }
-  const double b = block_sum(s, sh);
-  if (threadIdx.x == 0) gp(P.part)[(int64_t)region * P.pcap + lb] = b;
-  if (threadIdx.x == 0 && lb == 0) gp(P.pn)[region] = nblk;
+  smooth_part(P, x, lb, nblk, region, sh);
 }
 
 // pdot (pqdts.f90:589-614)

@@ -580,15 +580,17 @@ struct Engine {
   void L(const std::vector<int> &S, int x, bool recon, int ia = S_LA, int ib = S_LB, int ixmin = -1) {
     struct Scope sc(this, "L");
     if (recon) reconstruct(S, x);
+    // the loss product, and in the same launch the smoothing partials (region
+    // R_B): at least one smoothing block per problem, so that M = 1 still stores
+    // a zero partial
     GemmOps ops{BAND_F, x, OP_O, 0, 0, R_A};
+    ops.sm = R_B;
+    auto nsm = [&](int p) { return std::max(1, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); };
     hipEvent_t e0 = kbegin();
-    launch(S, [&](int p) { return nb_gemm(p, BAND_F); }, [&](const WorkList &wl, int tot) {
+    launch(S, [&](int p) { return nb_gemm(p, BAND_F) + nsm(p); }, [&](const WorkList &wl, int tot) {
       launch_gemm<double, B_PLAIN, E_LOSS, false, false>(tot, s, dprobs, wl, ops);
     });
     kend(e0, "spmm_F", S, [&](int p) { return gemm_work(p, BAND_F); });
-    // at least one block per problem, so that M = 1 still stores a zero partial
-    launch(S, [&](int p) { return std::max(1, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); },
-           [&](const WorkList &wl, int tot) { kb_smooth<<<tot, BLOCK, 0, s>>>(dprobs, wl, x, R_B); });
     launch(S, [](int) { return 1; }, [&](const WorkList &wl, int) {
       if (ixmin >= 0)
         kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 3, R_A, ia, R_B, ib, R_C, ixmin, 4);
