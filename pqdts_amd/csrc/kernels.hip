@@ -934,14 +934,6 @@ __global__ __launch_bounds__(BLOCK) void kb_conv_rows(const Prob *__restrict__ p
 // Element kernels over the M x N arrays: blocks of BLOCK x ELEMS elements.
 // ---------------------------------------------------------------------------
 
-// Smoothing part of the objective alone (smooth_part).
-__global__ __launch_bounds__(BLOCK) void kb_smooth(const Prob *__restrict__ probs, WorkList wl, int x,
-                                                   int region) {
-  __shared__ double sh[WAVES_PER_BLOCK];
-  BATCH_PROLOGUE();
-  smooth_part(P, x, lb, nblk, region, sh);
-}
-
 // pdot (pqdts.f90:589-614)
 __global__ __launch_bounds__(BLOCK) void kb_dot(const Prob *__restrict__ probs, WorkList wl, int x, int y,
                                                 int region) {
