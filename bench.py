@@ -718,8 +718,9 @@ def main():
     global HVP_SPLIT
     HVP_SPLIT = args.hvp_split
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems shard "
-              f"one per GPU with no communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems would "
+              f"shard one per GPU with no communication, so N GPUs serve N times the problems of one",
+              file=sys.stderr)
         sys.exit(2)
     import pqdts_amd
     from pqdts_amd import workloads
