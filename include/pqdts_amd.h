@@ -230,6 +230,16 @@ int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *co
  * x must not alias y, as in the reference. */
 int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n);
 
+/* The reference's only foreign symbol, same name, signature and contract
+ * (condat_simplexproj.c:60, called from pqdts.f90:1133 through an implicit
+ * Fortran interface, hence the trailing underscore and by-reference length):
+ * projects y[0..*plength) onto the simplex into x; y is not modified, x is the
+ * output and also the scratch, so x must not overlap y. Computed on the GPU
+ * with Condat's filter in the reference's operation order: x is bit-identical
+ * to the reference's. No status return (the reference has none): a failure is
+ * printed to stderr and aborts. */
+void simplexproj_condat_(double *y, double *x, int *plength);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,6 @@ The compute path is libpqdts_amd.so (HIP kernels for gfx950 behind the C ABI
 in include/pqdts_amd.h). Python is a thin ctypes layer over it.
 """
 from ._lib import (COL_MAJOR, ROW_MAJOR, SLOTS, Batch, Context, PqdtsError, device_count,  # noqa: F401
-                   device_cu_count, load, simplexproj_rows)
+                   device_cu_count, load, simplexproj_condat, simplexproj_rows)
 
-__all__ = ["Batch", "Context", "PqdtsError", "device_count", "device_cu_count", "load", "simplexproj_rows", "SLOTS", "ROW_MAJOR", "COL_MAJOR"]
+__all__ = ["Batch", "Context", "PqdtsError", "device_count", "device_cu_count", "load", "simplexproj_rows", "simplexproj_condat", "SLOTS", "ROW_MAJOR", "COL_MAJOR"]

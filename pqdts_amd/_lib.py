@@ -29,7 +29,7 @@ EXPORTS = [
     "pqdts_kernel_timing", "pqdts_kernel_stats", "pqdts_info", "pqdts_set_hvp_path", "pqdts_set_hvp_split",
     "pqdts_batch_create", "pqdts_batch_destroy", "pqdts_batch_admit", "pqdts_batch_step", "pqdts_batch_result",
     "pqdts_batch_release", "pqdts_batch_active", "pqdts_batch_kernel_timing", "pqdts_batch_kernel_stats",
-    "pqdts_device_cu_count", "pqdts_batch_set_cu_mask",
+    "pqdts_device_cu_count", "pqdts_batch_set_cu_mask", "simplexproj_condat_",
 ]
 
 _lib = None
@@ -83,6 +83,8 @@ def load():
     L.pqdts_solve.argtypes = [vp, i32, i32, d, i32, i32, EVENT_CB, vp, dp]
     L.pqdts_timers.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp]
     L.pqdts_simplexproj_rows.argtypes = [dp, dp, i64, i64]
+    L.simplexproj_condat_.argtypes = [dp, dp, C.POINTER(C.c_int)]
+    L.simplexproj_condat_.restype = None
     L.pqdts_kernel_timing.argtypes = [vp, i32]
     L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp, dp]
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
@@ -125,6 +127,27 @@ def simplexproj_rows(y: np.ndarray) -> np.ndarray:
     x = np.empty_like(y)
     _check(load().pqdts_simplexproj_rows(_p(y), _p(x), y.shape[0], y.shape[1]))
     return x
+
+
+def simplexproj_condat(y: np.ndarray) -> np.ndarray:
+    """One row through the reference's own entry point, simplexproj_condat_(y, x, &n)
+    (condat_simplexproj.c:60): bit-identical to the reference's result."""
+    y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+    x = np.empty_like(y)
+    n = C.c_int(y.size)
+    load().simplexproj_condat_(_p(y), _p(x), C.byref(n))
+    return x
+
+
+def cu_mask_words(cus, ncu: int = 0):
+    """(uint32 array, word count) with bit c set for every CU c in `cus`, covering at
+    least ncu CUs, so that every CU of the device has an explicit bit."""
+    cus = list(cus)
+    n = max(max(cus) + 1 if cus else 1, ncu) // 32 + 1
+    words = (C.c_uint32 * n)()
+    for c in cus:
+        words[c // 32] |= 1 << (c % 32)
+    return words, n
 
 
 def _slot(s):
@@ -400,12 +423,9 @@ class Batch:
         return int(load().pqdts_batch_active(self.h))
 
     def set_cus(self, cus):
-        """Restricts this (empty) batch to the compute units listed in `cus`."""
-        n = max(cus) // 32 + 1
-        words = (C.c_uint32 * n)()
-        for c in cus:
-            words[c // 32] |= 1 << (c % 32)
-        _check(load().pqdts_batch_set_cu_mask(self.h, words, n))
+        """Restricts this (empty) batch to the compute units listed in `cus`, which
+        must form one consecutive range (the library refuses other masks)."""
+        _check(load().pqdts_batch_set_cu_mask(self.h, *cu_mask_words(cus, device_cu_count())))
 
     def kernel_timing(self, on: bool):
         _check(load().pqdts_batch_kernel_timing(self.h, int(bool(on))))

@@ -899,6 +899,69 @@ __global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ pro
   }
 }
 
+// Condat's simplex projection (condat_simplexproj.c:60-95) of one row, in the
+// reference's summation order so that tau is bit-identical, one lane per row: the
+// filter is sequential. `x` (the output row) doubles as the candidate list, as
+// in the reference: candidates parked by a restart of the online filter sit in
+// x[0, base), the live candidates in x[base, base + nlive).
+__device__ void condat_row(const GLOBAL double *__restrict__ y, GLOBAL double *x, int n) {
+  // pass 1 (lines 66-75): online filter with restarts
+  int base = 0, nlive = 1;
+  x[0] = y[0];
+  double tau = y[0] - 1.0;
+  for (int i = 1; i < n; i++) {
+    const double v = y[i];
+    if (!(v > tau)) continue;
+    x[base + nlive] = v;
+    tau += (v - tau) / (double)(nlive + 1);
+    if (tau <= v - 1.0) {
+      // restart: everything before v is parked, v alone is live
+      tau = v - 1.0;
+      base += nlive;
+      nlive = 1;
+    } else {
+      nlive++;
+    }
+  }
+  // pass 2 (lines 76-82): parked values, newest first, re-enter in front of the
+  // live list when above tau
+  int head = base;
+  for (int k = base - 1; k >= 0; k--) {
+    const double v = x[k];
+    if (v > tau) {
+      x[--head] = v;
+      nlive++;
+      tau += (v - tau) / (double)nlive;
+    }
+  }
+  GLOBAL double *live = x + head;
+  // pass 3 (lines 83-89): sweeps dropping values <= tau until none is dropped
+  for (;;) {
+    const int last = nlive - 1;
+    int kept = 0;
+    for (int i = 0; i <= last; i++) {
+      const double v = live[i];
+      if (v > tau)
+        live[kept++] = v;
+      else
+        tau += (tau - v) / (double)(last - i + kept);
+    }
+    const bool dropped = kept <= last;
+    nlive = kept;
+    if (!dropped) break;
+  }
+  // lines 90-91
+  for (int i = 0; i < n; i++) x[i] = y[i] > tau ? y[i] - tau : 0.0;
+}
+
+// rows x n values from y into x, one lane per row (x must not alias y)
+__global__ __launch_bounds__(BLOCK) void kb_condat_rows(const double *__restrict__ y, double *x, int64_t rows,
+                                                        int n) {
+  const int64_t r = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (r >= rows) return;
+  condat_row(gp(y) + r * n, gp(x) + r * n, n);
+}
+
 // conv_test rows (pqdts.f90:1055-1060): lambda_i = max(0, -min_n G[i,n]), partial
 // of sum ((G + lambda) X)^2; with `store`, lambda and min(G+lambda) per row.
 __global__ __launch_bounds__(BLOCK) void kb_conv_rows(const Prob *__restrict__ probs, WorkList wl, int x, int g,

@@ -9,8 +9,9 @@ Fixtures
   cfg1_ref_prefix.json    reference stdout events of the config-1 solve (eps 1e-6, 1 thread)
                           up to the end of the first stage-2 Newton iteration: the part of the
                           trajectory that the reference reproduces across thread counts
-  cfg1_ref_states.npz     X after stage 1 (eps 1e-6, 1 thread) and the final X at eps 1e-11 with
-                          1 and 4 threads, with their objective values
+  cfg1_ref_states.npz     X after stage 1 (eps 1e-6, 1 thread) and the final X at eps 1e-11 and
+                          eps 1e-13 with 1 and 4 threads, with their objective values (at 1e-13
+                          the reference's 1-vs-4-thread spread is below 1e-8)
   condat_vectors.npz      inputs and outputs of the reference's condat_simplexproj.c
   driver_ref.json         reference stdout events of the drop-in cases of driver_cases.py
   driver_ref_states.npz   and the final X they wrote (rank_     0_final.dat)
@@ -87,15 +88,17 @@ def main():
             json.dump({"cmd_eps": 1e-6, "threads": 1, "events": prefix_events(ev)}, f)
         _, _, _, _, _, X1 = fortio.read_state(os.path.join(w, "rank_     0_oiter     1.dat"))
     states = {"stage1_eps1e-6": X1}
-    for th in (1, 4):
-        with tempfile.TemporaryDirectory() as w:
-            out = run_ref(prob, w, 1e-11, 2000, th)
-            ev = reftrace.parse(out)
-            its = [v for k, v in ev if k == reftrace.EV_ITER]
-            _, _, _, _, _, Xf = fortio.read_state(os.path.join(w, "rank_     0_final.dat"))
-            states[f"final_eps1e-11_t{th}"] = Xf
-            states[f"final_eps1e-11_t{th}_L"] = np.array(its[-1][2])
-            states[f"final_eps1e-11_t{th}_iters"] = np.array([len(its)])
+    for eps in (1e-11, 1e-13):
+        for th in (1, 4):
+            with tempfile.TemporaryDirectory() as w:
+                out = run_ref(prob, w, eps, 2000, th)
+                ev = reftrace.parse(out)
+                its = [v for k, v in ev if k == reftrace.EV_ITER]
+                _, _, _, _, _, Xf = fortio.read_state(os.path.join(w, "rank_     0_final.dat"))
+                tag = f"final_eps{eps:g}_t{th}"
+                states[tag] = Xf
+                states[f"{tag}_L"] = np.array(its[-1][2])
+                states[f"{tag}_iters"] = np.array([len(its)])
     np.savez_compressed(os.path.join(HERE, "cfg1_ref_states.npz"), **states)
 
     rng = np.random.default_rng(7)

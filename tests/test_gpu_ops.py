@@ -144,6 +144,23 @@ def test_simplexproj_rows_vs_reference_vectors():
         assert np.max(np.abs(got - x)) <= 1e-13 * max(1.0, np.max(np.abs(y))), i
 
 
+def test_simplexproj_condat_symbol_bit_exact_vs_reference_vectors():
+    """The reference's own foreign function, simplexproj_condat_(y, x, &n)
+    (condat_simplexproj.c:60), through ctypes with its by-reference signature:
+    x bit-identical to the reference build's output on every golden vector
+    (random, ties, all-negative, n = 1, n = 4097), y left unchanged."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "condat_vectors.npz"))
+    n = len([k for k in z.files if k.startswith("y")])
+    assert n >= 30
+    for i in range(n):
+        y, x = z[f"y{i}"].copy(), z[f"x{i}"]
+        y0 = y.copy()
+        got = pqdts_amd.simplexproj_condat(y)
+        assert np.array_equal(got, x), (i, float(np.max(np.abs(got - x))))
+        assert np.array_equal(y, y0)
+
+
 def test_doproj2_conv_transform(pair):
     ctx, orc, M, N = pair
     rng = np.random.default_rng(4)

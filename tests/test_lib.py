@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_symbols():
     txt = open(os.path.join(ROOT, "include", "pqdts_amd.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(pqdts_[A-Za-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b((?:pqdts|simplexproj)_[A-Za-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -52,3 +52,24 @@ def test_coo_files_roundtrip(tmp_path):
     F = np.zeros((prob.D, prob.M))
     F[r, c] = v
     assert np.array_equal(F, prob.F.toarray())
+
+
+def test_cu_mask_must_be_one_consecutive_range():
+    """pqdts_batch_set_cu_mask checks the mask before anything else, with no GPU
+    call: masks whose CUs do not form one consecutive range are refused (a host
+    crash was once observed while such a stream was being set up)."""
+    L = pqdts_amd.load()
+    words, n = _lib.cu_mask_words([0, 1, 2, 8, 9], 256)
+    assert n == 9
+    assert L.pqdts_batch_set_cu_mask(None, words, n) != 0
+    assert b"consecutive" in L.pqdts_last_error()
+    words, n = _lib.cu_mask_words(range(0, 256, 8), 256)
+    assert L.pqdts_batch_set_cu_mask(None, words, n) != 0
+    assert b"consecutive" in L.pqdts_last_error()
+    words, n = _lib.cu_mask_words([], 256)
+    assert L.pqdts_batch_set_cu_mask(None, words, n) != 0
+    assert b"no CU" in L.pqdts_last_error()
+    # a consecutive range passes the mask check and only then meets the null batch
+    words, n = _lib.cu_mask_words(range(32, 64), 256)
+    assert L.pqdts_batch_set_cu_mask(None, words, n) != 0
+    assert b"null batch" in L.pqdts_last_error()
