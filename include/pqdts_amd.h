@@ -225,6 +225,51 @@ int pqdts_batch_kernel_timing(pqdts_batch *b, int on);
 int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *counts, double *ms,
                              double *flops, double *bytes);
 
+/* ---- request server (served form; no counterpart in the reference) --------
+ * Queues requests (a problem as COO F and P, as pqdts_create takes them, with its
+ * solve parameters) and solves them on the GPU with continuous batching: each of
+ * `nlanes` size classes (by M*N: lane g holds lane_bounds[g-1] <= M*N <
+ * lane_bounds[g]) is one batch on its own stream, advanced by its own host thread.
+ * A lane admits the head of its queue (FIFO) when a slot is free and the
+ * request's device footprint fits the budget: the bytes pqdts_footprint reports
+ * (persistent + transient) are held from admission until the context exists,
+ * then its actual bytes until it finishes; a finished problem's Pi is copied to
+ * the host and its device memory freed. The device bytes of all admitted
+ * problems together never exceed budget_bytes (0 = 90 % of the free device
+ * memory at creation; each lane's own engine arrays, about 2.4 KB per slot, are
+ * not counted). lane_cus (may be NULL): per lane a consecutive CU range
+ * lo, hi, or -1, -1 for all CUs. A request larger than the budget is refused at
+ * submission. Every problem computes the same bits as solved alone. */
+typedef struct pqdts_server pqdts_server;
+int pqdts_server_create(int nlanes, const int64_t *lane_bounds, const int32_t *lane_cus, int capacity,
+                        int64_t budget_bytes, pqdts_server **out);
+/* stops the lanes (unfinished requests are dropped) and frees everything */
+int pqdts_server_destroy(pqdts_server *s);
+/* copies the request; X0 (row-major M x N) may be NULL for Pi = 1/N; mode,
+ * maxiter, eps as pqdts_solve (from stage 1); precision as pqdts_batch_admit */
+int pqdts_server_submit(pqdts_server *s, int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *F_row,
+                        const int32_t *F_col, const double *F_val, int64_t nP, const int32_t *P_row,
+                        const int32_t *P_col, const double *P_val, double gamma, int mode, int maxiter,
+                        double eps, int precision, const double *X0, int64_t *id);
+/* ids of requests finished since the last call (at most cap); waits up to
+ * timeout_ms (-1: until one finishes or none is left) when none is ready.
+ * Returns the count, or -1. */
+int pqdts_server_wait(pqdts_server *s, int64_t *ids, int cap, int timeout_ms);
+/* result of a finished request, which is then forgotten: X (row-major M x N,
+ * may be NULL), conv, stats[4] (finished, Newton iterations, PCG iterations,
+ * line-search trials), times[3] seconds (queued, admitted to done, total).
+ * Returns the request's own status (nonzero if its solve failed). */
+int pqdts_server_result(pqdts_server *s, int64_t id, double *X, double *conv, int *stats, double *times);
+/* info[0..7] = queued, running, finished not yet waited for, device bytes held
+ * now, peak device bytes held, budget, submitted, completed */
+int pqdts_server_info(pqdts_server *s, int64_t *info);
+/* Device bytes a problem will hold (persistent: its arrays, band packs, Gram
+ * matrix, mixed-precision copies, Gram-split partials) and, on top, while its
+ * context is created (transient: the dense matrices of the Gram formation), from
+ * F's pattern alone; what the server's admission counts. */
+int pqdts_footprint(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *F_row, const int32_t *F_col,
+                    int precision, int64_t *persistent, int64_t *transient);
+
 /* simplexproj_condat_ (condat_simplexproj.c:60) over many rows at once: rows of
  * length n from y (host, row-major) projected onto the simplex into x (host).
  * x must not alias y, as in the reference. */

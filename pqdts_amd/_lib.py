@@ -30,6 +30,8 @@ EXPORTS = [
     "pqdts_batch_create", "pqdts_batch_destroy", "pqdts_batch_admit", "pqdts_batch_step", "pqdts_batch_result",
     "pqdts_batch_release", "pqdts_batch_active", "pqdts_batch_kernel_timing", "pqdts_batch_kernel_stats",
     "pqdts_device_cu_count", "pqdts_batch_set_cu_mask", "simplexproj_condat_",
+    "pqdts_server_create", "pqdts_server_destroy", "pqdts_server_submit",
+    "pqdts_server_wait", "pqdts_server_result", "pqdts_server_info", "pqdts_footprint",
 ]
 
 _lib = None
@@ -85,6 +87,15 @@ def load():
     L.pqdts_simplexproj_rows.argtypes = [dp, dp, i64, i64]
     L.simplexproj_condat_.argtypes = [dp, dp, C.POINTER(C.c_int)]
     L.simplexproj_condat_.restype = None
+    i64p = C.POINTER(i64)
+    L.pqdts_server_create.argtypes = [i32, i64p, i32p, i32, i64, C.POINTER(vp)]
+    L.pqdts_server_destroy.argtypes = [vp]
+    L.pqdts_server_submit.argtypes = [vp, i64, i64, i64, i64, i32p, i32p, dp, i64, i32p, i32p, dp, d, i32, i32, d,
+                                      i32, dp, i64p]
+    L.pqdts_server_wait.argtypes = [vp, i64p, i32, i32]
+    L.pqdts_server_result.argtypes = [vp, i64, dp, dp, C.POINTER(i32), dp]
+    L.pqdts_server_info.argtypes = [vp, i64p]
+    L.pqdts_footprint.argtypes = [i64, i64, i64, i64, i32p, i32p, i32, i64p, i64p]
     L.pqdts_kernel_timing.argtypes = [vp, i32]
     L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp, dp]
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
@@ -441,3 +452,92 @@ class Batch:
                 if s not in done:
                     done[s] = self.result(s)
         return done
+
+
+def _coo(A):
+    import scipy.sparse as sps
+    c = sps.coo_matrix(A)
+    return (np.ascontiguousarray(c.row, np.int32), np.ascontiguousarray(c.col, np.int32),
+            np.ascontiguousarray(c.data, np.float64))
+
+
+def coo_of(prob):
+    """(F, P) of a problem as the int32 / float64 COO triples the C ABI takes."""
+    return _coo(prob.F), _coo(prob.P)
+
+
+def footprint(prob, precision=0):
+    """(persistent, transient) device bytes the problem will hold (pqdts_footprint)."""
+    fr, fc, _ = _coo(prob.F)
+    a, b = C.c_int64(), C.c_int64()
+    _check(load().pqdts_footprint(prob.M, prob.N, prob.D, fr.size, _p(fr, C.c_int32), _p(fc, C.c_int32),
+                                  int(precision), C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+class Server:
+    """The library's request server (pqdts_server_*): size-class lanes with continuous
+    batching and admission against a device-byte budget."""
+
+    def __init__(self, lane_bounds=(), lane_cus=None, capacity=256, budget_bytes=0):
+        nl = len(lane_bounds) + 1
+        b = (C.c_int64 * max(1, len(lane_bounds)))(*lane_bounds)
+        cus = None
+        if lane_cus is not None:
+            flat = []
+            for r in lane_cus:
+                flat += [-1, -1] if r is None else [int(r[0]), int(r[-1])]
+            cus = (C.c_int32 * len(flat))(*flat)
+        h = C.c_void_p()
+        _check(load().pqdts_server_create(nl, b, cus, int(capacity), int(budget_bytes), C.byref(h)))
+        self.h = h
+        self.shape = {}
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            load().pqdts_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, prob, mode=2, maxiter=5000, eps=None, precision=0, X0=None, coo=None) -> int:
+        """coo: (F COO, P COO) as prepared by coo_of(prob), to keep the conversion out of
+        the submission path."""
+        (fr, fc, fv), (pr, pc, pv) = coo if coo is not None else coo_of(prob)
+        x0 = None if X0 is None else np.ascontiguousarray(X0, np.float64)
+        rid = C.c_int64()
+        _check(load().pqdts_server_submit(
+            self.h, prob.M, prob.N, prob.D, fv.size, _p(fr, C.c_int32), _p(fc, C.c_int32), _p(fv), pv.size,
+            _p(pr, C.c_int32), _p(pc, C.c_int32), _p(pv), float(prob.gamma), int(mode), int(maxiter),
+            float(prob.eps if eps is None else eps), int(precision), None if x0 is None else _p(x0),
+            C.byref(rid)))
+        self.shape[rid.value] = (prob.M, prob.N)
+        return rid.value
+
+    def wait(self, timeout_ms=-1, cap=4096):
+        buf = (C.c_int64 * cap)()
+        n = load().pqdts_server_wait(self.h, buf, cap, int(timeout_ms))
+        if n < 0:
+            _check(-1)
+        return [buf[i] for i in range(n)]
+
+    def result(self, rid):
+        M, N = self.shape.pop(rid)
+        X = np.empty((M, N))
+        conv = C.c_double()
+        stats = (C.c_int * 4)()
+        times = (C.c_double * 3)()
+        _check(load().pqdts_server_result(self.h, int(rid), _p(X), C.byref(conv), stats, times))
+        return {"X": X, "conv": conv.value, "finished": bool(stats[0]), "newton": stats[1], "pcg": stats[2],
+                "trials": stats[3], "queued_s": times[0], "run_s": times[1], "latency_s": times[2]}
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 8)()
+        _check(load().pqdts_server_info(self.h, a))
+        keys = ("queued", "running", "finished_unclaimed", "bytes_in_use", "peak_bytes", "budget", "submitted",
+                "completed")
+        return dict(zip(keys, list(a)))

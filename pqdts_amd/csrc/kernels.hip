@@ -820,10 +820,10 @@ __global__ __launch_bounds__(BLOCK) void kb_transform(const Prob *__restrict__ p
 //                  implicit entry; per-block partial of xmin = min(0, min_j t[j,maxr]);
 //                  with `recon_again` the objective's own rebuild (pqdts.f90:929-935)
 //                  follows.
-__global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ probs, WorkList wl, int t,
-                                                    int form_trial, int recon_again, int region) {
-  __shared__ double sh[WAVES_PER_BLOCK];
-  ROW_PROLOGUE();
+__device__ __forceinline__ void project_block(const Prob &P, int lb, int nblk, bool full, double arg, int t,
+                                              int form_trial, int recon_again, int region, double *sh) {
+  const int j = lb * WAVES_PER_BLOCK + wave_id();
+  const int N = P.N;
   double xm = 0.0;
   if (j < P.M) {
     GLOBAL double *row = gp(P.op[t]) + (int64_t)j * N;
@@ -897,6 +897,13 @@ __global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ pro
     gp(P.part)[(int64_t)region * P.pcap + lb] = v;
     if (lb == 0) gp(P.pn)[region] = nblk;
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ probs, WorkList wl, int t,
+                                                    int form_trial, int recon_again, int region) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  BATCH_PROLOGUE();
+  project_block(P, lb, nblk, full, arg, t, form_trial, recon_again, region, sh);
 }
 
 // Condat's simplex projection (condat_simplexproj.c:60-95) of one row, in the

@@ -19,8 +19,8 @@
 #include <functional>
 #include <limits>
 #include <map>
-#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1126,23 +1126,6 @@ struct HostBand {
   std::vector<int> klo, nch;
 };
 
-template <class Span>
-int64_t band_elems(int R, Span span) {
-  int64_t total = 0;
-  for (int t = 0; t < cdiv(R, TILE); t++) {
-    int lo = INT32_MAX, hi = -1;
-    for (int r = t * TILE; r < std::min(R, (t + 1) * TILE); r++) {
-      const auto sp = span(r);
-      if (sp.second > 0) {
-        lo = std::min(lo, sp.first);
-        hi = std::max(hi, sp.first + sp.second - 1);
-      }
-    }
-    if (hi >= 0) total += (int64_t)cdiv(hi - lo + 1, KC) * KC * TILE;
-  }
-  return total;
-}
-
 template <class Span, class Val>
 HostBand pack_band(int R, Span span, Val val) {
   const int nt = cdiv(R, TILE);
@@ -1280,14 +1263,20 @@ void ensure_gram(pqdts_ctx *c) {
 // more than the shorter chain saves. An explicit split applies to both. The
 // split depends on the problem and its precision alone, so a problem computes
 // the same bits alone and in a batch.
+int split_by_shape(int64_t N, int gram_tiles, int gram_nch) {
+  int hs = 1;
+  const int64_t ntile = (int64_t)cdiv(N, TILE) * gram_tiles;
+  for (int k = 2; k <= 4; k *= 2)
+    if (k * ntile <= 1024 && gram_nch >= 6 * k) hs = k;
+  return hs;
+}
+
 void apply_gram_split(pqdts_ctx *c) {
   int hs = 1;
   if (c->hs_pref > 0) {
     hs = c->hs_pref;
   } else if (c->bg.pack) {
-    const int ntile = cdiv(c->N, TILE) * c->bg.ntiles, mx = c->gram_nch;
-    for (int k = 2; k <= 4; k *= 2)
-      if (k * ntile <= 1024 && mx >= 6 * k) hs = k;
+    hs = split_by_shape(c->N, c->bg.ntiles, c->gram_nch);
   }
   // the partials are only needed once G exists
   const bool need = hs > 1 && c->bg.pack;
@@ -1331,14 +1320,151 @@ void destroy_ctx(pqdts_ctx *c) {
   delete c;
 }
 
-pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *Fr, const int32_t *Fc,
-                      const double *Fv, int64_t nP, const int32_t *Pr, const int32_t *Pc, const double *Pv,
-                      double gamma) {
+// The shape of a problem as its context will hold it, from the COO pattern of F
+// alone: the row spans of F (the reference's Frows, pqdts.f90:1328-1364, built in
+// O(nnz) with zero fill), the column spans (rows of F^T), the row bands of
+// G = F^T F, the band tiles of each, the Hessian-product path and the Gram split.
+struct Plan {
+  int64_t M = 0, N = 0, D = 0;
+  std::vector<int> lo, len, kdlo, kdhi, glo, gw;
+  std::vector<uint8_t> has;
+  std::vector<int64_t> off;
+  int64_t nspan = 0, gram_nnz = 0;
+  int64_t bf_elems = 0, bt_elems = 0, bg_elems = 0;
+  int bf_tiles = 0, bt_tiles = 0, bg_tiles = 0, bg_nch = 0;
+  bool gram_auto = false;
+};
+
+void check_shape(int64_t M, int64_t N, int64_t D, int64_t nF, int64_t nP) {
   require(M >= 1 && N >= 1 && D >= 1, "need M >= 1, N >= 1, D >= 1");
   require(M * N < (int64_t)INT32_MAX * 8 && D * N < (int64_t)INT32_MAX * 8 && M < INT32_MAX && N < INT32_MAX &&
               D < INT32_MAX,
           "problem too large");
   require(nF >= 0 && nP >= 0, "negative nnz");
+}
+
+// elements and most chunks of one tile of a band stored per tile of TILE rows
+template <class Span>
+std::pair<int64_t, int> band_shape(int R, Span span) {
+  int64_t total = 0;
+  int mx = 0;
+  for (int t = 0; t < cdiv(R, TILE); t++) {
+    int lo = INT32_MAX, hi = -1;
+    for (int r = t * TILE; r < std::min(R, (t + 1) * TILE); r++) {
+      const auto sp = span(r);
+      if (sp.second > 0) {
+        lo = std::min(lo, sp.first);
+        hi = std::max(hi, sp.first + sp.second - 1);
+      }
+    }
+    if (hi >= 0) {
+      const int nch = cdiv(hi - lo + 1, KC);
+      total += (int64_t)nch * KC * TILE;
+      mx = std::max(mx, nch);
+    }
+  }
+  return {total, mx};
+}
+
+Plan make_plan(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *Fr, const int32_t *Fc) {
+  check_shape(M, N, D, nF, 0);
+  require(nF == 0 || (Fr && Fc), "null F arrays");
+  Plan p;
+  p.M = M;
+  p.N = N;
+  p.D = D;
+  p.lo.assign(D, INT32_MAX);
+  p.len.assign(D, 0);
+  p.has.assign(D, 0);
+  std::vector<int> hi(D, -1);
+  for (int64_t e = 0; e < nF; e++) {
+    const int d = Fr[e], k = Fc[e];
+    require(d >= 0 && d < D && k >= 0 && k < M, "F index out of range at entry " + std::to_string(e));
+    p.lo[d] = std::min(p.lo[d], k);
+    hi[d] = std::max(hi[d], k);
+    p.has[d] = 1;
+  }
+  p.off.assign(D + 1, 0);
+  for (int d = 0; d < D; d++) {
+    if (!p.has[d]) p.lo[d] = 0;
+    p.len[d] = p.has[d] ? hi[d] - p.lo[d] + 1 : 0;
+    p.off[d + 1] = p.off[d] + p.len[d];
+  }
+  p.nspan = p.off[D];
+  p.kdlo.assign(M, INT32_MAX);
+  p.kdhi.assign(M, -1);
+  p.glo.assign(M, INT32_MAX);
+  p.gw.assign(M, 0);
+  std::vector<int> ghi(M, -1);
+  for (int d = 0; d < D; d++)
+    for (int k = p.lo[d]; k < p.lo[d] + p.len[d]; k++) {
+      p.kdlo[k] = std::min(p.kdlo[k], d);
+      p.kdhi[k] = std::max(p.kdhi[k], d);
+      p.glo[k] = std::min(p.glo[k], p.lo[d]);
+      ghi[k] = std::max(ghi[k], p.lo[d] + p.len[d] - 1);
+    }
+  for (int k = 0; k < M; k++) {
+    if (p.kdhi[k] < 0) p.kdlo[k] = 0;
+    if (ghi[k] < 0) p.glo[k] = 0;
+    p.gw[k] = ghi[k] >= 0 ? ghi[k] - p.glo[k] + 1 : 0;
+    p.gram_nnz += p.gw[k];
+  }
+  p.bf_tiles = cdiv(D, TILE);
+  p.bt_tiles = p.bg_tiles = cdiv(M, TILE);
+  p.bf_elems = band_shape((int)D, [&](int d) { return std::make_pair(p.lo[d], p.len[d]); }).first;
+  p.bt_elems = band_shape((int)M, [&](int k) {
+                 return std::make_pair(p.kdlo[k], p.kdhi[k] >= 0 ? p.kdhi[k] - p.kdlo[k] + 1 : 0);
+               }).first;
+  const auto g = band_shape((int)M, [&](int k) { return std::make_pair(p.glo[k], p.gw[k]); });
+  p.bg_elems = g.first;
+  p.bg_nch = g.second;
+  // the Gram HVP when its band tiles clearly cost less than the two F products
+  p.gram_auto = (double)p.bg_elems < 0.8 * (double)(p.bf_elems + p.bt_elems) && M * M * 8 < ((int64_t)64 << 30);
+  return p;
+}
+
+// Device bytes a context holds: every array its creation, the Gram matrix, mixed
+// precision and the Gram split allocate (the stream and host copies aside).
+int64_t footprint_bytes(int64_t M, int64_t N, int64_t D, int64_t nspan, int64_t bf_elems, int bf_tiles,
+                        int64_t bt_elems, int bt_tiles, bool gram, int64_t bg_elems, int bg_tiles, bool fp32,
+                        int hs) {
+  const int64_t MN = M * N, DN = D * N;
+  const int64_t pcap = std::max<int64_t>({(int64_t)cdiv(N, TILE) * std::max(bf_tiles, bt_tiles),
+                                          cdiv(M, WAVES_PER_BLOCK), cdiv(MN, EBLOCK), 1});
+  int64_t b = D * (4 + 4 + 8 + 1) + 8 * nspan + M * (4 + 4) + 16 * DN + 48 * MN + M * (8 + 8 + 4) + MN +
+              M * 4 * 8 + 8 * pcap * NREG + 8 * NSCAL + 4 * NREG + (int64_t)sizeof(Prob);
+  // a band pack holds at least one element
+  auto band = [&](int64_t elems, int tiles) { return 8 * std::max<int64_t>(elems, 1) + 16 * (int64_t)tiles; };
+  b += band(bf_elems, bf_tiles) + band(bt_elems, bt_tiles) + (gram ? band(bg_elems, bg_tiles) : 0);
+  if (fp32)
+    b += 4 * (std::max<int64_t>(bf_elems, 1) + std::max<int64_t>(bt_elems, 1) +
+              (gram ? std::max<int64_t>(bg_elems, 1) : 0)) +
+         4 * MN + 8 * M;
+  if (hs > 1 && gram) b += 8 * (int64_t)hs * MN;
+  return b;
+}
+
+// bytes held only while the Gram matrix is formed: dense F (D x M), dense G (M x M)
+int64_t gram_transient_bytes(int64_t M, int64_t D) { return 8 * (D * M + M * M) + (int64_t)sizeof(Prob); }
+
+// what a context of this plan will hold (persistent) and, on top, while it is created
+void plan_footprint(const Plan &p, int hvp_pref, int hs_pref, bool fp32, int64_t *persistent, int64_t *transient) {
+  const bool gram = hvp_pref == 2 || (hvp_pref == 0 && p.gram_auto);
+  const int hs = !gram ? 1 : hs_pref > 0 ? hs_pref : split_by_shape(p.N, p.bg_tiles, p.bg_nch);
+  *persistent = footprint_bytes(p.M, p.N, p.D, p.nspan, p.bf_elems, p.bf_tiles, p.bt_elems, p.bt_tiles, gram,
+                                p.bg_elems, p.bg_tiles, fp32, hs);
+  *transient = gram ? gram_transient_bytes(p.M, p.D) : 0;
+}
+
+int env_hvp_pref() {
+  const char *e = getenv("PQDTS_HVP");
+  return e ? ((e[0] == 'G') ? 2 : (e[0] == 'F') ? 1 : 0) : 0;
+}
+
+pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *Fr, const int32_t *Fc,
+                      const double *Fv, int64_t nP, const int32_t *Pr, const int32_t *Pc, const double *Pv,
+                      double gamma) {
+  check_shape(M, N, D, nF, nP);
   require(nF == 0 || (Fr && Fc && Fv), "null F arrays");
   require(nP == 0 || (Pr && Pc && Pv), "null P arrays");
   int ndev = 0;
@@ -1346,37 +1472,12 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
 
   // F ingest: O(nnz) span build with zero fill and last-wins duplicates (the same
   // spans as the O(D*nnz) scan of pqdts.f90:1328-1364)
-  std::vector<int> lo(D, INT32_MAX), hi(D, -1), len(D, 0);
-  std::vector<uint8_t> has(D, 0);
-  for (int64_t e = 0; e < nF; e++) {
-    const int d = Fr[e], k = Fc[e];
-    require(d >= 0 && d < D && k >= 0 && k < M, "F index out of range at entry " + std::to_string(e));
-    lo[d] = std::min(lo[d], k);
-    hi[d] = std::max(hi[d], k);
-    has[d] = 1;
-  }
-  std::vector<int64_t> off(D + 1, 0);
-  for (int d = 0; d < D; d++) {
-    if (!has[d]) lo[d] = 0;
-    len[d] = has[d] ? hi[d] - lo[d] + 1 : 0;
-    off[d + 1] = off[d] + len[d];
-  }
+  const Plan pl = make_plan(M, N, D, nF, Fr, Fc);
+  const std::vector<int> &lo = pl.lo, &len = pl.len, &kdlo = pl.kdlo, &kdhi = pl.kdhi, &glo = pl.glo, &gw = pl.gw;
+  const std::vector<uint8_t> &has = pl.has;
+  const std::vector<int64_t> &off = pl.off;
   std::vector<double> val(off[D], 0.0);
   for (int64_t e = 0; e < nF; e++) val[off[Fr[e]] + (Fc[e] - lo[Fr[e]])] = Fv[e];
-  // column spans of F (rows of F^T) and the row bands of G = F^T F
-  std::vector<int> kdlo(M, INT32_MAX), kdhi(M, -1), glo(M, INT32_MAX), ghi(M, -1), gw(M, 0);
-  for (int d = 0; d < D; d++)
-    for (int k = lo[d]; k < lo[d] + len[d]; k++) {
-      kdlo[k] = std::min(kdlo[k], d);
-      kdhi[k] = std::max(kdhi[k], d);
-      glo[k] = std::min(glo[k], lo[d]);
-      ghi[k] = std::max(ghi[k], lo[d] + len[d] - 1);
-    }
-  for (int k = 0; k < M; k++) {
-    if (kdhi[k] < 0) kdlo[k] = 0;
-    if (ghi[k] < 0) glo[k] = 0;
-    gw[k] = ghi[k] >= 0 ? ghi[k] - glo[k] + 1 : 0;
-  }
   HostBand hbf = pack_band(
       (int)D, [&](int d) { return std::make_pair(lo[d], len[d]); },
       [&](int d, int k) { return val[off[d] + (k - lo[d])]; });
@@ -1404,12 +1505,10 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     c->nspan = off[D];
     c->glo = glo;
     c->gw = gw;
-    for (int w : gw) c->gram_nnz += w;
-    c->gram_elems = band_elems((int)M, [&](int k) { return std::make_pair(glo[k], gw[k]); });
-    // the Gram HVP when its band tiles clearly cost less than the two F products
-    const int64_t fpath = (int64_t)hbf.pack.size() + (int64_t)hbt.pack.size();
-    c->gram_auto = (double)c->gram_elems < 0.8 * (double)fpath && M * M * 8 < ((int64_t)64 << 30);
-    if (const char *e = getenv("PQDTS_HVP")) c->hvp_pref = (e[0] == 'G') ? 2 : (e[0] == 'F') ? 1 : 0;
+    c->gram_nnz = pl.gram_nnz;
+    c->gram_elems = pl.bg_elems;
+    c->gram_auto = pl.gram_auto;
+    c->hvp_pref = env_hvp_pref();
     HIPC(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     c->flo = dalloc<int>(D);
     c->flen = dalloc<int>(D);
@@ -1476,6 +1575,12 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     throw;
   }
   return c;
+}
+
+// device bytes the context holds now (the footprint formula on its actual arrays)
+int64_t ctx_device_bytes(const pqdts_ctx *c) {
+  return footprint_bytes(c->M, c->N, c->D, c->nspan, c->bf.elems, c->bf.ntiles, c->bt.elems, c->bt.ntiles,
+                         c->bg.pack != nullptr, c->bg.elems, c->bg.ntiles, c->p32 != nullptr, c->hw ? c->hs : 1);
 }
 
 pqdts_ctx *checked(pqdts_ctx *c) {
@@ -1546,6 +1651,41 @@ struct pqdts_batch {
   Engine eng;
   std::vector<int> finished;  // slots finished since the last query
 };
+
+namespace {
+// Places ctx (its X is the starting point) in a free slot of e with a fresh solver
+// state; returns the slot.
+int engine_admit(Engine &e, pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage, int output,
+                 int precision, pqdts_event_cb cb, void *user) {
+  require(mode == 1 || mode == 2, "mode must be 1 or 2");
+  require(start_stage == 1 || start_stage == 2, "start_stage must be 1 or 2");
+  require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
+  require(ctx->batch == nullptr, "the context is already in a batch (release it first)");
+  int p = -1;
+  for (int i = 0; i < e.cap; i++)
+    if (!e.ctx[i]) {
+      p = i;
+      break;
+    }
+  require(p >= 0, "batch is full");
+  HIPC(hipStreamSynchronize(ctx->s));
+  if (precision) ensure_fp32(ctx);
+  e.fp32[p] = precision;
+  e.attach(p, ctx);
+  ctx->batch = &e;
+  PState *q = new PState();
+  q->mode = mode;
+  q->maxiter = maxiter;
+  q->eps = eps;
+  q->start_stage = start_stage;
+  q->output = output;
+  q->cb = cb;
+  q->user = user;
+  e.st[p].reset(q);
+  ctx->full = true;
+  return p;
+}
+}  // namespace
 
 // ===========================================================================
 // C ABI
@@ -1901,9 +2041,7 @@ int pqdts_info(pqdts_ctx *ctx, int64_t *info) {
     info[9] = ctx->gram_nnz;
     info[10] = ctx->bt.elems;
     // device bytes held by this problem
-    info[11] = 8 * (6 * ctx->MN + 2 * ctx->DN + ctx->nspan + ctx->bf.elems + ctx->bt.elems + ctx->bg.elems) +
-               4 * (ctx->bf.pack32 ? ctx->bf.elems + ctx->bt.elems + ctx->bg.elems : 0) + ctx->MN +
-               (ctx->hw ? 8 * ctx->hs * ctx->MN : 0) + (ctx->p32 ? 4 * ctx->MN : 0);
+    info[11] = ctx_device_bytes(ctx);
     info[12] = ctx->hs;
   });
 }
@@ -2034,34 +2172,7 @@ int pqdts_batch_admit(pqdts_batch *b, pqdts_ctx *ctx, int mode, int maxiter, dou
                       int output, int precision, pqdts_event_cb cb, void *user, int *slot) {
   return guarded([&] {
     require(b != nullptr, "null batch");
-    checked(ctx);
-    require(mode == 1 || mode == 2, "mode must be 1 or 2");
-    require(start_stage == 1 || start_stage == 2, "start_stage must be 1 or 2");
-    require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (mixed)");
-    require(ctx->batch == nullptr, "the context is already in a batch (release it first)");
-    Engine &e = b->eng;
-    int p = -1;
-    for (int i = 0; i < e.cap; i++)
-      if (!e.ctx[i]) {
-        p = i;
-        break;
-      }
-    require(p >= 0, "batch is full");
-    HIPC(hipStreamSynchronize(ctx->s));
-    if (precision) ensure_fp32(ctx);
-    e.fp32[p] = precision;
-    e.attach(p, ctx);
-    ctx->batch = &e;
-    PState *q = new PState();
-    q->mode = mode;
-    q->maxiter = maxiter;
-    q->eps = eps;
-    q->start_stage = start_stage;
-    q->output = output;
-    q->cb = cb;
-    q->user = user;
-    e.st[p].reset(q);
-    ctx->full = true;
+    const int p = engine_admit(b->eng, checked(ctx), mode, maxiter, eps, start_stage, output, precision, cb, user);
     if (slot) *slot = p;
   });
 }
@@ -2155,3 +2266,5 @@ int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *co
 }
 
 }  // extern "C"
+
+#include "server.hip"

@@ -233,3 +233,45 @@ def test_cu_restricted_batch_matches_alone():
     assert r["conv"] == conva
     with pytest.raises(pqdts_amd.PqdtsError):
         b.set_cus(range(4))  # only an empty batch can be restricted
+
+
+# Problems of the benchmark mix (workloads.batch_mix(256), by index) whose optimum FP64
+# determines at eps = 1e-11: two FP64 solves that differ only in the Hessian product's
+# summation order agree there to <= 6e-7 (asserted below). Small gamma or large M leave
+# Pi undetermined far beyond 1e-6 at any eps FP64 can reach, for FP64 itself as much as
+# for mixed precision, so the 1e-6 bar is asserted where it is meaningful.
+DETERMINED_MIX = [4, 10, 13, 34, 35, 45, 49, 94, 102, 139, 206, 225]
+
+
+def test_mixed_precision_meets_1e6_on_mix_problems():
+    """Batched mixed precision (FP32 Hessian products inside CG, FP64 everywhere else)
+    against batched FP64 (bit-identical to the plain solver alone) at eps = 1e-11 on 12
+    problems of the uneven mix: every problem converged in both, ||dPi||inf <= 1e-6 and
+    relative objective gap <= 1e-9; the FP64-vs-FP64 spread is within the same bar."""
+    allp = workloads.batch_mix(256, eps=1e-11)
+    probs = [allp[i] for i in DETERMINED_MIX]
+    eps = 1e-11
+
+    def run(prec, path="auto"):
+        b = pqdts_amd.Batch(len(probs))
+        ctxs = []
+        for p in probs:
+            c = pqdts_amd.Context.from_problem(p)
+            c.set_hvp_path(path)
+            start(c, p)
+            ctxs.append(c)
+            b.admit(c, 2, 5000, eps, 1, 0, prec, record=False)
+        res = b.run()
+        return [c.get("X") for c in ctxs], [c.L("X") for c in ctxs], [res[i] for i in range(len(probs))]
+
+    X64, L64, r64 = run(0)
+    Xm, Lm, rm = run(1)
+    XF, _, rF = run(0, "F")
+    for k, p in enumerate(probs):
+        assert r64[k]["conv"] < eps and rm[k]["conv"] < eps and rF[k]["conv"] < eps, p.name
+        dpi = float(np.max(np.abs(X64[k] - Xm[k])))
+        floor = float(np.max(np.abs(X64[k] - XF[k])))
+        gap = abs(Lm[k] - L64[k]) / abs(L64[k])
+        print(f"{p.name} M={p.M} N={p.N} gamma={p.gamma:.1e}: mixed ||dPi||inf {dpi:.2e}, "
+              f"fp64-vs-fp64 {floor:.2e}, gap {gap:.1e}")
+        assert dpi <= 1e-6 and floor <= 1e-6 and gap <= 1e-9, p.name
