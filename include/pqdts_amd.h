@@ -275,6 +275,19 @@ int pqdts_footprint(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t *
  * x must not alias y, as in the reference. */
 int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n);
 
+/* pqdts_simplexproj_rows without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). y and x are
+ * DEVICE pointers to rows x n doubles (row-major); x == y projects in place,
+ * otherwise y is copied to x first and must not overlap it. The work is
+ * enqueued on `stream` (a hipStream_t; NULL = the default stream) and the call
+ * returns without waiting. The projection's per-block partials live in a static
+ * device array of the library (2 MiB per GPU, held until the process ends), so
+ * at most 1048576 rows per call; larger calls fail with an error. Calls that
+ * overlap on one GPU race on that array (its contents, never the projected
+ * rows). Same kernel arithmetic as pqdts_simplexproj_rows: bit-identical rows. */
+int pqdts_simplexproj_rows_noalloc(const double *y, double *x, int64_t rows, int64_t n, void *stream);
+
 /* The reference's only foreign symbol, same name, signature and contract
  * (condat_simplexproj.c:60, called from pqdts.f90:1133 through an implicit
  * Fortran interface, hence the trailing underscore and by-reference length):

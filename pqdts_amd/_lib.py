@@ -30,7 +30,7 @@ EXPORTS = [
     "pqdts_batch_create", "pqdts_batch_destroy", "pqdts_batch_admit", "pqdts_batch_step", "pqdts_batch_result",
     "pqdts_batch_release", "pqdts_batch_active", "pqdts_batch_kernel_timing", "pqdts_batch_kernel_stats",
     "pqdts_device_cu_count", "pqdts_batch_set_cu_mask", "simplexproj_condat_",
-    "pqdts_server_create", "pqdts_server_destroy", "pqdts_server_submit",
+    "pqdts_simplexproj_rows_noalloc", "pqdts_server_create", "pqdts_server_destroy", "pqdts_server_submit",
     "pqdts_server_wait", "pqdts_server_result", "pqdts_server_info", "pqdts_footprint",
 ]
 
@@ -87,6 +87,7 @@ def load():
     L.pqdts_simplexproj_rows.argtypes = [dp, dp, i64, i64]
     L.simplexproj_condat_.argtypes = [dp, dp, C.POINTER(C.c_int)]
     L.simplexproj_condat_.restype = None
+    L.pqdts_simplexproj_rows_noalloc.argtypes = [vp, vp, i64, i64, vp]
     i64p = C.POINTER(i64)
     L.pqdts_server_create.argtypes = [i32, i64p, i32p, i32, i64, C.POINTER(vp)]
     L.pqdts_server_destroy.argtypes = [vp]

@@ -906,6 +906,14 @@ __global__ __launch_bounds__(BLOCK) void kb_project(const Prob *__restrict__ pro
   project_block(P, lb, nblk, full, arg, t, form_trial, recon_again, region, sh);
 }
 
+// The full-space projection of the rows of operand 0 with the descriptor passed
+// by value (kernel arguments), for callers that keep no descriptor in device
+// memory: one block per WAVES_PER_BLOCK rows, the same arithmetic as kb_project.
+__global__ __launch_bounds__(BLOCK) void kb_project_byval(const Prob P, int region) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  project_block(P, blockIdx.x, gridDim.x, true, 0.0, 0, 0, 0, region, sh);
+}
+
 // Condat's simplex projection (condat_simplexproj.c:60-95) of one row, in the
 // reference's summation order so that tau is bit-identical, one lane per row: the
 // filter is sequential. `x` (the output row) doubles as the candidate list, as

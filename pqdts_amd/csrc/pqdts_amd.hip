@@ -2112,6 +2112,63 @@ int pqdts_simplexproj_rows(const double *y, double *x, int64_t rows, int64_t n) 
   });
 }
 
+}  // extern "C"
+
+// ---- the projection without allocation ------------------------------------
+// pqdts_simplexproj_rows allocates a device copy of the rows, a descriptor and the
+// projection's per-block partials on every call. The variant below takes the rows
+// in device memory and a stream, passes the descriptor as kernel arguments and
+// keeps the partials (one double per block of WAVES_PER_BLOCK rows) and their
+// count in this file-scope device array. Each GPU holds its own copy until the
+// process ends; its address is looked up once per device.
+namespace {
+constexpr int64_t NOALLOC_MAX_ROWS = (int64_t)1 << 20;
+constexpr int64_t NOALLOC_PARTS = NOALLOC_MAX_ROWS / WAVES_PER_BLOCK;
+constexpr int NOALLOC_MAX_DEV = 64;
+static __device__ __align__(256) double scratch[NOALLOC_PARTS + NREG];
+std::mutex g_scratch_mu;
+double *g_scratch[NOALLOC_MAX_DEV] = {};
+
+// this device's address of `scratch` (the name itself is a host-side stand-in)
+double *scratch_on_current_device() {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  require(dev >= 0 && dev < NOALLOC_MAX_DEV, "device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "no device address for the scratch array");
+    g_scratch[dev] = (double *)p;
+  }
+  return g_scratch[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int pqdts_simplexproj_rows_noalloc(const double *y, double *x, int64_t rows, int64_t n, void *stream) {
+  return guarded([&] {
+    require(rows >= 0 && n >= 1 && n < INT32_MAX && rows * n < (int64_t)INT32_MAX * 8, "bad shape");
+    require(rows <= NOALLOC_MAX_ROWS, "pqdts_simplexproj_rows_noalloc: at most 1048576 rows per call");
+    require(y != nullptr && x != nullptr, "null device pointer");
+    require(x == y || x + rows * n <= y || y + rows * n <= x, "x and y overlap without being the same rows");
+    if (rows == 0) return;
+    double *sp = scratch_on_current_device();
+    hipStream_t s = (hipStream_t)stream;
+    if (x != y) HIPC(hipMemcpyAsync(x, y, sizeof(double) * rows * n, hipMemcpyDeviceToDevice, s));
+    Prob p{};
+    p.M = (int)rows;
+    p.N = (int)n;
+    p.pcap = cdiv(rows, WAVES_PER_BLOCK);
+    p.op[0] = x;
+    p.part = sp;
+    p.pn = (int *)(sp + NOALLOC_PARTS);
+    kb_project_byval<<<p.pcap, BLOCK, 0, s>>>(p, 0);
+    HIPC(hipGetLastError());
+  });
+}
+
 // The reference's own foreign function (condat_simplexproj.c:60), called per row
 // by its doproj (pqdts.f90:1133) through an implicit interface: all arguments by
 // reference, y read only, x written (and used as scratch), no aliasing, no
