@@ -240,7 +240,7 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, timing, tag, lane
         if lane_cus and len(groups) > 1 and lane_cus[g] is not None:
             b.set_cus(lane_cus[g])
         if timing:
-            b.kernel_timing(True)
+            b.kernel_timing(timing)
         batches.append(b)
     lat, res = [None] * len(problems), [None] * len(problems)
     kst, errors = {}, []
@@ -363,6 +363,37 @@ def measured_traffic(kernel):
     return None
 
 
+def determined_subset(args, pqdts_amd, problems, ctxs, eps=1e-11):
+    """The problems of the mix whose optimum FP64 pins (M <= 260, gamma >= 3e-4) solved
+    to eps = 1e-11: batched mixed precision against the plain FP64 solver one problem at
+    a time, with the 1e-6 gate on every problem."""
+    import dataclasses
+    idx = [i for i, p in enumerate(problems) if p.M <= 260 and p.gamma >= 3e-4]
+    if not idx:
+        return None
+    sub = [dataclasses.replace(problems[i], eps=eps) for i in idx]
+    sctx = [ctxs[i] for i in idx]
+    dt_m, lat_m, X_m, res_m, _ = run_batched(pqdts_amd, sub, sctx, 1, args.maxiter, False, "determined, mixed")
+    t_plain, X_p, conv_p = 0.0, [], []
+    for p, c in zip(sub, sctx):
+        c.pset(1.0 / p.N, "X")
+        t0 = time.perf_counter()
+        conv, _ = c.solve(2, args.maxiter, eps, 1, 0)
+        t_plain += time.perf_counter() - t0
+        X_p.append(c.get("X"))
+        conv_p.append(conv < eps)
+    dpi = [float(np.max(np.abs(a - b))) for a, b in zip(X_m, X_p)]
+    gated = sum(1 for r, d, cp in zip(res_m, dpi, conv_p) if r["conv"] < eps and cp and d <= 1e-6)
+    rate_m, rate_p = gated / dt_m, sum(conv_p) / t_plain
+    log(f"determined subset: {len(sub)} problems at eps {eps:g}: mixed {dt_m:.2f} s, {gated} within 1e-6; "
+        f"plain fp64 {t_plain:.2f} s")
+    return {"problems": len(sub), "selection": "M <= 260 and gamma >= 3e-4", "eps": eps,
+            "mixed_within_1e-6": gated, "mixed_seconds": dt_m, "mixed_solves_per_s_gated": rate_m,
+            "plain_fp64_seconds": t_plain, "plain_fp64_solves_per_s": rate_p,
+            "speedup_vs_plain_fp64": rate_m / rate_p if rate_p > 0 else None,
+            "dpi_inf_max": max(dpi), "dpi_inf_median": pct(dpi, 50), "p99_latency_s": pct(lat_m, 99)}
+
+
 def workload_mix(args, pqdts_amd, workloads):
     T_START = time.time()
     problems = workloads.batch_mix(args.count, eps=args.eps)
@@ -389,7 +420,7 @@ def workload_mix(args, pqdts_amd, workloads):
     runs, kst_m = [], {}
     t_timed = time.perf_counter()
     for s in range(args.steps):
-        dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, True,
+        dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, args.ksample,
                                             f"batched mixed step {s}", args.lanes, lane_cus)
         for name, v in kst.items():
             old = kst_m.get(name, (0, 0.0, 0.0, 0.0))
@@ -404,6 +435,17 @@ def workload_mix(args, pqdts_amd, workloads):
     steps = len(runs)
     total = sum(r[0] for r in runs)
     dt_m, lat_m, X_m, res_m = runs[0]
+    if args.legs == "timed":
+        # only the timed region (for profiler runs): the line carries the timing and the roofline
+        rf = roofline_of(kst_m)
+        return {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "steps": steps,
+                "warmup": args.warmup, "ms_per_step": 1e3 * total / steps, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32 Hessian products / f64 elsewhere",
+                "data": "synthetic", "config": {"workload": "mix timed region only (--legs timed)",
+                                                "count": len(problems), "eps": args.eps},
+                "converged_solves_per_s_ungated": steps * sum(r["conv"] < p.eps for r, p in zip(res_m, problems))
+                / total, "roofline": rf, "cpu_baseline": None,
+                "calibration_copy_bytes": calibration_copy(pqdts_amd, problems, ctxs) if args.calib_copy else None}
     Lm = []
     for c, x in zip(ctxs, X_m):
         c.put("X", x)
@@ -453,6 +495,7 @@ def workload_mix(args, pqdts_amd, workloads):
     for c in sctx:
         c.set_hvp_path("auto")
     floor = [float(np.max(np.abs(X_b64[i] - x))) for i, x in zip(seq_idx, X_alt)]
+    det = determined_subset(args, pqdts_amd, problems, ctxs)
     nconv_m = sum(conv_m)
     value = steps * sum(gated) / total
     seq_lat = np.cumsum([seq_t[i] for i in seq_idx]) * args.seq_stride
@@ -499,6 +542,7 @@ def workload_mix(args, pqdts_amd, workloads):
             "note": "fp64_vs_fp64 = ||dPi||inf between two FP64 solves of the same problem whose Hessian products "
                     "sum in a different order (Gram vs F products): how far eps leaves Pi undetermined; the 1e-6 "
                     "gate cannot hold where FP64 itself does not"},
+        "determined_subset": det,
         "slowest_mixed": [{"M": problems[i].M, "N": problems[i].N, "gamma": problems[i].gamma,
                            "newton": res_m[i]["newton"], "pcg": res_m[i]["pcg"], "latency_s": lat_m[i]}
                           for i in sorted(range(len(problems)), key=lambda k: -lat_m[k])[:5]],
@@ -775,6 +819,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--legs", choices=["all", "timed"], default="all",
+                    help="mix: all measurements, or only the warmup and timed steps (profiler runs)")
+    ap.add_argument("--ksample", type=int, default=16,
+                    help="per-kernel HIP-event timing in the timed steps: every k-th operator launch")
     ap.add_argument("--budget", type=float, default=100.0,
                     help="wall-clock seconds for the timed steps (mix): steps run while they fit")
     ap.add_argument("--kernels", type=int, default=0)

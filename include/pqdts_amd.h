@@ -141,13 +141,25 @@ int pqdts_lalpha_min_auglag(pqdts_ctx *ctx, double *alpha, double *dlalpha);
 int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage, int output,
                 pqdts_event_cb cb, void *user, double *conv);
 
-/* Device-side timing of the named sections of the last solve (names as the
- * reference's timers: "L", "dL", "BF", "Lalpha_min", ...). Returns the number of
- * timers; fills up to `cap` entries. */
+/* The named sections timed on this context so far, under the reference's timer
+ * names (pqdts.f90:103-186): min, L, L_O, dL, dL_O, dL_dLdX, BF, BF_O, BF_BX, CG,
+ * active_dLdX, Lalpha_min, doproj, doproj2, pdot and the BLAS-1 names. Sections
+ * the reference times but this library fuses into a neighbouring kernel
+ * (precond1/2 into the CG kernels, active into the product's epilogue, L_gam,
+ * dL_gam, BF_gam into the products, Bdiag computed once per context) are not
+ * listed. Seconds are host wall-clock time around the asynchronous launches
+ * unless pqdts_device_timers is on. Returns the number of timers; fills up to
+ * `cap` entries. */
 int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *seconds);
+/* on != 0: time every section on the device, between HIP events recorded on the
+ * context's stream at its start and end (the device time of the section's
+ * launches); 0: host wall clock. Costs two event records per section. */
+int pqdts_device_timers(pqdts_ctx *ctx, int on);
 
 /* Per-kernel device timing with HIP events on the context's stream: `on`
- * enables it and clears the totals; pqdts_kernel_stats returns the number of
+ * enables it (1: every launch; k > 1: every k-th operator launch, a sample of
+ * the durations whose flops and bytes are those of the sampled launches; 0: off)
+ * and clears the totals; pqdts_kernel_stats returns the number of
  * kernel names and fills launch counts and total milliseconds. Band products:
  * "spmm_F" = O = F*V, "spmmT_F" = 2 F^T W + stencil, "hvp_G" = 2 G p with the
  * Gram matrix (names ending in 32: the FP32 mixed-precision variants).

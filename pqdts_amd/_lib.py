@@ -31,7 +31,7 @@ EXPORTS = [
     "pqdts_batch_release", "pqdts_batch_active", "pqdts_batch_kernel_timing", "pqdts_batch_kernel_stats",
     "pqdts_device_cu_count", "pqdts_batch_set_cu_mask", "simplexproj_condat_",
     "pqdts_simplexproj_rows_noalloc", "pqdts_server_create", "pqdts_server_destroy", "pqdts_server_submit",
-    "pqdts_server_wait", "pqdts_server_result", "pqdts_server_info", "pqdts_footprint",
+    "pqdts_server_wait", "pqdts_server_result", "pqdts_server_info", "pqdts_footprint", "pqdts_device_timers",
 ]
 
 _lib = None
@@ -98,6 +98,7 @@ def load():
     L.pqdts_server_info.argtypes = [vp, i64p]
     L.pqdts_footprint.argtypes = [i64, i64, i64, i64, i32p, i32p, i32, i64p, i64p]
     L.pqdts_kernel_timing.argtypes = [vp, i32]
+    L.pqdts_device_timers.argtypes = [vp, i32]
     L.pqdts_kernel_stats.argtypes = [vp, i32, C.c_void_p, C.POINTER(i32), dp, dp, dp]
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
     L.pqdts_set_hvp_path.argtypes = [vp, i32]
@@ -339,13 +340,18 @@ class Context:
         """Split of the Gram product over band chunks: 0 by shape, 1 none, 2..16 fixed."""
         _check(load().pqdts_set_hvp_split(self.h, int(split)))
 
-    def kernel_timing(self, on: bool):
-        _check(load().pqdts_kernel_timing(self.h, int(bool(on))))
+    def kernel_timing(self, on):
+        """True / 1: time every launch; k > 1: every k-th launch; False / 0: off."""
+        _check(load().pqdts_kernel_timing(self.h, int(on)))
 
     def kernel_stats(self) -> dict:
         """{kernel: (launches, total_ms, algorithmic flops, algorithmic bytes)} from HIP events on the
         context's stream."""
         return _kstats(load().pqdts_kernel_stats, self.h)
+
+    def device_timers(self, on: bool):
+        """Section timers measure device time (HIP events on the context's stream)."""
+        _check(load().pqdts_device_timers(self.h, int(bool(on))))
 
     def timers(self):
         L = load()
@@ -439,8 +445,9 @@ class Batch:
         must form one consecutive range (the library refuses other masks)."""
         _check(load().pqdts_batch_set_cu_mask(self.h, *cu_mask_words(cus, device_cu_count())))
 
-    def kernel_timing(self, on: bool):
-        _check(load().pqdts_batch_kernel_timing(self.h, int(bool(on))))
+    def kernel_timing(self, on):
+        """True / 1: time every launch; k > 1: every k-th launch; False / 0: off."""
+        _check(load().pqdts_batch_kernel_timing(self.h, int(on)))
 
     def kernel_stats(self) -> dict:
         return _kstats(load().pqdts_batch_kernel_stats, self.h)

@@ -278,8 +278,12 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 // B operand: as stored, with the implicit column expanded (reduced problems), minus
 // P, or the FP32 expanded copy of the CG direction (P.p32, written where the
 // direction is formed)
-enum BMode { B_PLAIN = 0, B_EXPAND = 1, B_DIFF = 2, B_P32 = 3 };
-enum EMode { E_STORE = 0, E_LOSS = 1, E_STORE_LOSS = 2, E_GRAD = 3, E_HVP = 4, E_PART = 5 };
+// B_SPAN: B is F itself (rows d, columns k) read from its span storage, zero
+// outside each row's span: the Gram matrix F^T F formed without a dense copy of F.
+enum BMode { B_PLAIN = 0, B_EXPAND = 1, B_DIFF = 2, B_P32 = 3, B_SPAN = 4 };
+// E_GBAND: the product is the Gram matrix, stored straight into G's band pack
+// (P.bg); output tiles outside the band of their row tile are not computed.
+enum EMode { E_STORE = 0, E_LOSS = 1, E_STORE_LOSS = 2, E_GRAD = 3, E_HVP = 4, E_PART = 5, E_GBAND = 6 };
 enum BandSel { BAND_F = 0, BAND_FT = 1, BAND_G = 2 };
 
 struct GemmOps {
@@ -368,6 +372,10 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
   const int nch = EM == E_PART ? ((sp + 1) * nch_t) / P.hs - c0 : nch_t;
   const GLOBAL T *ap = gp(Mf::pack(band)) + gp(band.off)[tile] + (int64_t)c0 * (KC * TR);
   const int k0 = gp(band.klo)[tile] + c0 * KC;
+  // E_GBAND: columns [gk0, gk0 + gkw) of row tile `tile` form its band of G
+  const int gk0 = EM == E_GBAND ? gp(P.bg.klo)[tile] : 0;
+  const int gkw = EM == E_GBAND ? gp(P.bg.nch)[tile] * KC : 0;
+  if (EM == E_GBAND && (n0 + TC <= gk0 || n0 >= gk0 + gkw)) return;  // uniform over the block
 
   T ra[AE], rb[BE];
   auto gload = [&](int c) {
@@ -382,6 +390,9 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
       double v = 0.0;
       if (BM == B_P32 && k < nk && n < N) {
         v = (double)gp(P.p32)[(int64_t)k * N + n];
+      } else if (BM == B_SPAN && k < nk && n < N) {
+        const int rel = n - gp(P.flo)[k];
+        if (rel >= 0 && rel < gp(P.flen)[k]) v = gp(P.fval)[gp(P.foff)[k] + rel];
       } else if (k < nk && n < N) {
         const int64_t idx = (int64_t)k * N + n;
         v = Bp[idx];
@@ -548,6 +559,8 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
           const int64_t idx = (int64_t)row * N + n;
           const double v = FP32 ? acc64[i][j][q] : (double)acc[i][j][q];
           if (EM == E_STORE || EM == E_STORE_LOSS || EM == E_PART) Cp[idx] = v;
+          if (EM == E_GBAND && n >= gk0 && n < gk0 + gkw)
+            gp(const_cast<double *>(P.bg.pack))[gp(P.bg.off)[tile] + (int64_t)(n - gk0) * GT + (row - r0)] = v;
           if (LOSS && gp(P.fhas)[row]) {
             const double t = v - gp(P.P)[idx];
             redv = fma(t, t, redv);
@@ -696,20 +709,6 @@ __global__ __launch_bounds__(BLOCK) void kb_foldval_gram(const Prob *__restrict_
   const int k = lb * WAVES_PER_BLOCK + wave_id();
   if (k >= P.M) return;
   foldval_gram_row(P, gp(P.op[ddsrc]), k, kc, tile);
-}
-
-// Dense copy of F (D x M, row-major) into `out`, one wave per row: used once to
-// form the Gram matrix with the band GEMM.
-__global__ __launch_bounds__(BLOCK) void kb_dense_rows(const Prob *__restrict__ probs, WorkList wl,
-                                                       double *__restrict__ out) {
-  BATCH_PROLOGUE();
-  const int d = lb * WAVES_PER_BLOCK + wave_id();
-  if (d >= P.D) return;
-  GLOBAL double *row = gp(out) + (int64_t)d * P.M;
-  for (int k = lane_id(); k < P.M; k += WAVE) {
-    const int rel = k - gp(P.flo)[d];
-    row[k] = (rel >= 0 && rel < gp(P.flen)[d]) ? gp(P.fval)[gp(P.foff)[d] + rel] : 0.0;
-  }
 }
 
 // Bdiag (pqdts.f90:208-244), one wave per k: 2 sum_d F[d,k]^2 + 2 gamma [k<M-1]

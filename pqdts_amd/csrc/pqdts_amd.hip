@@ -104,6 +104,7 @@ struct Timer {
   int count = 0;
   double sec = 0.0;
   std::chrono::steady_clock::time_point t0;
+  std::vector<hipEvent_t> open;  // device timing: start events of the running sections
 };
 
 struct Engine;
@@ -241,12 +242,21 @@ struct Engine {
   double *dlog = nullptr;         // device-side CG logs [cap][CG_LOG_ROUNDS][5]
   double *hlog = nullptr;         // pinned host copy
   std::vector<int> fp32;          // per-slot: mixed precision Hessian products
-  // timers
+  // timers: host wall clock around the (asynchronous) launches, or with
+  // dev_timers the device time between HIP events recorded on the stream at the
+  // start and the end of each section
   std::map<std::string, Timer> timers;
   std::vector<std::string> timer_order;
-  bool sync_timers = false;
+  bool dev_timers = false;
+  struct TPending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<TPending> tpending;
   // per-kernel HIP-event timing
   bool ktiming = false;
+  int ksample = 1;    // time every ksample-th operator launch (a sample of the durations)
+  uint64_t kseq = 0;
   struct Work {
     double flops = 0.0, bytes = 0.0;
   };
@@ -293,7 +303,6 @@ struct Engine {
     HIPC(hipHostMalloc((void **)&hscal, sizeof(double) * NSCAL * cap, hipHostMallocDefault));
     dlog = dalloc<double>((int64_t)cap * CG_LOG_ROUNDS * 5);
     HIPC(hipHostMalloc((void **)&hlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipHostMallocDefault));
-    sync_timers = getenv("PQDTS_SYNC_TIMERS") && atoi(getenv("PQDTS_SYNC_TIMERS")) != 0;
     graphs_on = getenv("PQDTS_GRAPHS") && atoi(getenv("PQDTS_GRAPHS")) != 0;
   }
   void drop_graphs() {
@@ -360,7 +369,16 @@ struct Engine {
     st[slot].reset();
   }
 
-  // ---- timing
+  // ---- timing (never throws: tstop runs in destructors)
+  hipEvent_t tevent() {
+    if (!kpool.empty()) {
+      hipEvent_t e = kpool.back();
+      kpool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+  }
   void tstart(const char *name) {
     auto it = timers.find(name);
     if (it == timers.end()) {
@@ -368,12 +386,39 @@ struct Engine {
       timer_order.push_back(name);
     }
     it->second.count++;
-    it->second.t0 = std::chrono::steady_clock::now();
+    if (dev_timers) {
+      hipEvent_t a = tevent();
+      if (a && hipEventRecord(a, s) != hipSuccess) a = nullptr;
+      it->second.open.push_back(a);
+    } else {
+      it->second.t0 = std::chrono::steady_clock::now();
+    }
   }
   void tstop(const char *name) {
-    if (sync_timers) (void)hipStreamSynchronize(s);
     auto &t = timers[name];
-    t.sec += std::chrono::duration<double>(std::chrono::steady_clock::now() - t.t0).count();
+    if (!t.open.empty()) {
+      hipEvent_t a = t.open.back();
+      t.open.pop_back();
+      hipEvent_t b = a ? tevent() : nullptr;
+      if (b && hipEventRecord(b, s) == hipSuccess) {
+        tpending.push_back({name, a, b});
+      } else {
+        if (a) kpool.push_back(a);
+        if (b) kpool.push_back(b);
+      }
+    } else if (!dev_timers) {
+      t.sec += std::chrono::duration<double>(std::chrono::steady_clock::now() - t.t0).count();
+    }
+  }
+  // adds the device time of the finished sections (the stream has been synchronised)
+  void tresolve() {
+    for (auto &p : tpending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) timers[p.name].sec += ms / 1e3;
+      kpool.push_back(p.a);
+      kpool.push_back(p.b);
+    }
+    tpending.clear();
   }
   int kernel_id(const std::string &name) {
     for (size_t i = 0; i < knames.size(); i++)
@@ -397,6 +442,7 @@ struct Engine {
   }
   hipEvent_t kbegin() {
     if (!ktiming) return nullptr;
+    if (ksample > 1 && (kseq++ % (uint64_t)ksample) != 0) return nullptr;
     hipEvent_t a = kevent();
     HIPC(hipEventRecord(a, s));
     return a;
@@ -489,6 +535,7 @@ struct Engine {
     }
     HIPC(hipStreamSynchronize(s));
     if (!kpending.empty()) kfold();
+    if (!tpending.empty()) tresolve();
   }
 
   // ---- work lists: groups of at most MAXB slots; nb(slot) blocks each
@@ -587,9 +634,11 @@ struct Engine {
     ops.sm = R_B;
     auto nsm = [&](int p) { return std::max(1, cdiv((int64_t)(ctx[p]->M - 1) * ctx[p]->N, EBLOCK)); };
     hipEvent_t e0 = kbegin();
+    tstart("L_O");
     launch(S, [&](int p) { return nb_gemm(p, BAND_F) + nsm(p); }, [&](const WorkList &wl, int tot) {
       launch_gemm<double, B_PLAIN, E_LOSS, false, false>(tot, s, dprobs, wl, ops);
     });
+    tstop("L_O");
     kend(e0, "spmm_F", S, [&](int p) { return gemm_work(p, BAND_F); });
     launch(S, [](int) { return 1; }, [&](const WorkList &wl, int) {
       if (ixmin >= 0)
@@ -604,12 +653,16 @@ struct Engine {
   void dL(const std::vector<int> &S, int x, int g) {
     struct Scope sc(this, "dL");
     reconstruct(S, x);
+    tstart("dL_O");
     gemm_t<double, B_PLAIN, E_STORE, false, false>(S, GemmOps{BAND_F, x, OP_O, 0, 0, 0}, "spmm_F");
+    tstop("dL_O");
+    tstart("dL_dLdX");
     std::vector<int> R = reduced_of(S);
     if (!R.empty())
       launch(R, [&](int p) { return nb_rows(p); },
              [&](const WorkList &wl, int tot) { kb_foldval<T_GRAD><<<tot, BLOCK, 0, s>>>(dprobs, wl, x); });
     gemm_t<double, B_DIFF, E_GRAD, false, false>(S, GemmOps{BAND_FT, OP_O, g, x, 0, 0}, "spmmT_F");
+    tstop("dL_dLdX");
   }
 
   // BF (pqdts.f90:246-342): bx = H dd; with `dot` the partial dd.bx goes to S_V2.
@@ -653,6 +706,8 @@ struct Engine {
     const GemmOps hg{BAND_G, dd, bx, dd, dd, R_B};
     // B = p with its implicit columns expanded (reduced problems)
     std::vector<int> GS;
+    const bool any_gram = !g64.empty() || !g32.empty() || !g64s.empty() || !g32s.empty();
+    if (any_gram) tstart("BF_O");
     hvp_gemm<double, B_EXPAND>(g64, hg, mask, dot, "hvp_G");
     if (p32)
       hvp_gemm<float, B_P32>(g32, hg, mask, dot, "hvp_G32");
@@ -675,16 +730,20 @@ struct Engine {
       });
       kend(e0, "hvp_epi", GS, [&](int p) { return stream_work(p, 8.0 * (ctx[p]->hs + 3) + 1.0); });
     }
+    if (any_gram) tstop("BF_O");
     // F path: O = F p~, fold values, then 2 F^T O with the epilogue
     for (int pass = 0; pass < 2; pass++) {
       const std::vector<int> &FS = pass ? f32 : f64;
       if (FS.empty()) continue;
+      tstart("BF_O");
       if (pass && p32)
         gemm_t<float, B_P32, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F32");
       else if (pass)
         gemm_t<float, B_EXPAND, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F32");
       else
         gemm_t<double, B_EXPAND, E_STORE, false, false>(FS, GemmOps{BAND_F, dd, OP_O, 0, 0, 0}, "spmm_F");
+      tstop("BF_O");
+      tstart("BF_BX");
       std::vector<int> FR = reduced_of(early_of(FS));
       if (!FR.empty())
         launch(FR, [&](int p) { return nb_rows(p); },
@@ -694,6 +753,7 @@ struct Engine {
         hvp_gemm<float, B_PLAIN>(FS, ht, mask, dot, "spmmT_F32");
       else
         hvp_gemm<double, B_PLAIN>(FS, ht, mask, dot, "spmmT_F");
+      tstop("BF_BX");
     }
   }
   // split Gram product: hs partial products per problem (nothing fused)
@@ -1050,7 +1110,7 @@ void Engine::tick() {
           cg_dir(C, ib1, i_rz, j);
         }
       };
-      const bool graph = graphs_on && !private_scal && !ktiming && !sync_timers;
+      const bool graph = graphs_on && !private_scal && !ktiming && !dev_timers;
       for (int j0 = 1; j0 <= 50; j0 += 8) {
         const int j1 = std::min(j0 + 7, 50);
         if (graph)
@@ -1198,22 +1258,52 @@ void ensure_fp32(pqdts_ctx *c) {
 
 void apply_gram_split(pqdts_ctx *c);
 
-// Forms G = F^T F once, on the device: F is expanded to a dense D x M matrix and
-// multiplied by the packed F^T band with the band GEMM itself; the rows of G are
-// then packed into band tiles like F.
+// Forms G = F^T F once, on the device, straight into its band tiles: the band
+// GEMM multiplies F^T's band by F read from its span storage (B_SPAN), computes
+// only the output tiles that meet the band of their row tile, and stores them in
+// the band pack (E_GBAND). No dense copy of F or G exists at any point.
 void ensure_gram(pqdts_ctx *c) {
   if (c->bg.pack) return;
-  const int M = c->M, D = c->D;
-  double *Fd = nullptr, *Gd = nullptr;
+  const int M = c->M;
+  // band layout of G's rows (columns glo[k] .. glo[k] + gw[k] - 1), values 0
+  pqdts_ctx::DBand &g = c->bg;
+  const int nt = cdiv(M, TILE);
+  std::vector<int64_t> off(nt, 0);
+  std::vector<int> klo(nt, 0), nch(nt, 0);
+  int64_t total = 0;
+  int mx = 0;
+  for (int t = 0; t < nt; t++) {
+    int lo = INT32_MAX, hi = -1;
+    for (int k = t * TILE; k < std::min(M, (t + 1) * TILE); k++)
+      if (c->gw[k] > 0) {
+        lo = std::min(lo, c->glo[k]);
+        hi = std::max(hi, c->glo[k] + c->gw[k] - 1);
+      }
+    off[t] = total;
+    if (hi >= 0) {
+      klo[t] = lo;
+      nch[t] = cdiv(hi - lo + 1, KC);
+      total += (int64_t)nch[t] * KC * TILE;
+      mx = std::max(mx, nch[t]);
+    }
+  }
+  Prob *dd = nullptr;
   try {
-    Fd = dalloc<double>((int64_t)D * M);
-    Gd = dalloc<double>((int64_t)M * M);
-    // a temporary descriptor whose operands are the dense F (B) and G (C)
+    g.ntiles = nt;
+    g.elems = std::max<int64_t>(total, 1);
+    g.pack = dalloc<double>(g.elems);
+    g.off = dalloc<int64_t>(nt);
+    g.klo = dalloc<int>(nt);
+    g.nch = dalloc<int>(nt);
+    HIPC(hipMemsetAsync(g.pack, 0, sizeof(double) * g.elems, c->s));
+    HIPC(hipMemcpyAsync(g.off, off.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice, c->s));
+    HIPC(hipMemcpyAsync(g.klo, klo.data(), sizeof(int) * nt, hipMemcpyHostToDevice, c->s));
+    HIPC(hipMemcpyAsync(g.nch, nch.data(), sizeof(int) * nt, hipMemcpyHostToDevice, c->s));
+    // a temporary descriptor: N := M (the columns of G), the band of G as output
     Prob d = c->desc;
     d.N = M;
-    d.op[0] = Fd;
-    d.op[1] = Gd;
-    Prob *dd = dalloc<Prob>(1);
+    d.bg = to_band(g);
+    dd = dalloc<Prob>(1);
     HIPC(hipMemcpyAsync(dd, &d, sizeof(Prob), hipMemcpyHostToDevice, c->s));
     WorkList wl;
     wl.guard = -1;
@@ -1222,31 +1312,20 @@ void ensure_gram(pqdts_ctx *c) {
     wl.flags[0] = F_FULL;
     wl.a[0] = 0.0;
     wl.boff[0] = 0;
-    wl.boff[1] = cdiv(D, WAVES_PER_BLOCK);
-    kb_dense_rows<<<wl.boff[1], BLOCK, 0, c->s>>>(dd, wl, Fd);
-    HIPC(hipGetLastError());
     wl.boff[1] = cdiv(M, TILE) * c->bt.ntiles;
-    // G = F^T F: band F^T (rows k, columns d), B = dense F (D x M) with N := M
+    // G = F^T F: band F^T (rows k, columns d), B = F (rows d, columns l)
     GemmOps ops{BAND_FT, 0, 1, 0, 0, 0};
-    launch_gemm<double, B_PLAIN, E_STORE, false, false>(wl.boff[1], c->s, dd, wl, ops);
+    kb_band_gemm<double, KC, B_SPAN, E_GBAND, false, false, false><<<wl.boff[1], BLOCK, 0, c->s>>>(dd, wl, ops);
     HIPC(hipGetLastError());
-    std::vector<double> Gh((size_t)M * M);
-    HIPC(hipMemcpyAsync(Gh.data(), Gd, sizeof(double) * Gh.size(), hipMemcpyDeviceToHost, c->s));
     HIPC(hipStreamSynchronize(c->s));
-    (void)hipFree(dd);
-    HostBand h = pack_band(
-        M, [&](int k) { return std::make_pair(c->glo[k], c->gw[k]); },
-        [&](int k, int l) { return Gh[(size_t)k * M + l]; });
-    upload_band(c, c->bg, h);
-    c->gram_nch = 0;
-    for (int n : h.nch) c->gram_nch = std::max(c->gram_nch, n);
   } catch (...) {
-    if (Fd) (void)hipFree(Fd);
-    if (Gd) (void)hipFree(Gd);
+    if (dd) (void)hipFree(dd);
+    free_band(c->bg);
     throw;
   }
-  (void)hipFree(Fd);
-  (void)hipFree(Gd);
+  (void)hipFree(dd);
+  c->gram_nch = mx;
+  refresh_desc(c);
   if (c->bf.pack32) ensure_fp32(c);
   apply_gram_split(c);
 }
@@ -1444,8 +1523,8 @@ int64_t footprint_bytes(int64_t M, int64_t N, int64_t D, int64_t nspan, int64_t 
   return b;
 }
 
-// bytes held only while the Gram matrix is formed: dense F (D x M), dense G (M x M)
-int64_t gram_transient_bytes(int64_t M, int64_t D) { return 8 * (D * M + M * M) + (int64_t)sizeof(Prob); }
+// bytes held only while the Gram matrix is formed: a temporary descriptor
+int64_t gram_transient_bytes() { return (int64_t)sizeof(Prob); }
 
 // what a context of this plan will hold (persistent) and, on top, while it is created
 void plan_footprint(const Plan &p, int hvp_pref, int hs_pref, bool fp32, int64_t *persistent, int64_t *transient) {
@@ -1453,7 +1532,7 @@ void plan_footprint(const Plan &p, int hvp_pref, int hs_pref, bool fp32, int64_t
   const int hs = !gram ? 1 : hs_pref > 0 ? hs_pref : split_by_shape(p.N, p.bg_tiles, p.bg_nch);
   *persistent = footprint_bytes(p.M, p.N, p.D, p.nspan, p.bf_elems, p.bf_tiles, p.bt_elems, p.bt_tiles, gram,
                                 p.bg_elems, p.bg_tiles, fp32, hs);
-  *transient = gram ? gram_transient_bytes(p.M, p.D) : 0;
+  *transient = gram ? gram_transient_bytes() : 0;
 }
 
 int env_hvp_pref() {
@@ -1871,9 +1950,10 @@ int pqdts_pdot(pqdts_ctx *ctx, int x, int y, double *val) {
 }
 
 // BLAS-1 on slots; an operand of -1 is not used by that kind
-static int blas1(pqdts_ctx *ctx, int z, int x, int y, double a, double b, int kind) {
+static int blas1(pqdts_ctx *ctx, int z, int x, int y, double a, double b, int kind, const char *timer) {
   return guarded([&] {
     Engine &e = E1(ctx);
+    Engine::Scope sc(&e, timer);
     slot_ptr(ctx, z);
     if (x >= 0) slot_ptr(ctx, x);
     if (y >= 0) slot_ptr(ctx, y);
@@ -1882,18 +1962,22 @@ static int blas1(pqdts_ctx *ctx, int z, int x, int y, double a, double b, int ki
   });
 }
 
-int pqdts_paxpy(pqdts_ctx *ctx, double alpha, int x, int y) { return blas1(ctx, y, x, y, alpha, 0.0, AX_AXPY); }
+int pqdts_paxpy(pqdts_ctx *ctx, double alpha, int x, int y) {
+  return blas1(ctx, y, x, y, alpha, 0.0, AX_AXPY, "paxpy");
+}
 int pqdts_pgaxpy(pqdts_ctx *ctx, double alpha, int x, double beta, int y) {
-  return blas1(ctx, y, x, y, alpha, beta, AX_GAXPY);
+  return blas1(ctx, y, x, y, alpha, beta, AX_GAXPY, "gpaxpy");
 }
 int pqdts_pgaxpy2(pqdts_ctx *ctx, double alpha, int x, double beta, int y, int z) {
-  return blas1(ctx, z, x, y, alpha, beta, AX_GAXPY);
+  return blas1(ctx, z, x, y, alpha, beta, AX_GAXPY, "pgpaxpy2");
 }
-int pqdts_pset(pqdts_ctx *ctx, double alpha, int x) { return blas1(ctx, x, -1, -1, alpha, 0.0, AX_SET); }
-int pqdts_pcopy(pqdts_ctx *ctx, int x, int y) { return blas1(ctx, y, x, -1, 1.0, 0.0, AX_COPY); }
-int pqdts_pscale(pqdts_ctx *ctx, double alpha, int x) { return blas1(ctx, x, x, -1, alpha, 0.0, AX_SCALE); }
+int pqdts_pset(pqdts_ctx *ctx, double alpha, int x) { return blas1(ctx, x, -1, -1, alpha, 0.0, AX_SET, "pset"); }
+int pqdts_pcopy(pqdts_ctx *ctx, int x, int y) { return blas1(ctx, y, x, -1, 1.0, 0.0, AX_COPY, "pcopy"); }
+int pqdts_pscale(pqdts_ctx *ctx, double alpha, int x) {
+  return blas1(ctx, x, x, -1, alpha, 0.0, AX_SCALE, "pscale");
+}
 int pqdts_pcopy_scale(pqdts_ctx *ctx, double alpha, int x, int y) {
-  return blas1(ctx, y, x, -1, alpha, 0.0, AX_SCALE);
+  return blas1(ctx, y, x, -1, alpha, 0.0, AX_SCALE, "pcopy_scale");
 }
 
 int pqdts_transform(pqdts_ctx *ctx) {
@@ -1968,6 +2052,8 @@ int pqdts_timers(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double
   int n = 0;
   int rc = guarded([&] {
     Engine &e = E1(ctx);
+    HIPC(hipStreamSynchronize(e.s));
+    e.tresolve();
     for (const auto &name : e.timer_order) {
       if (n < cap) {
         const Timer &t = e.timers[name];
@@ -2006,12 +2092,23 @@ static void kernel_timing_of(Engine &e, int on) {
   HIPC(hipStreamSynchronize(e.s));
   e.kfold();
   e.ktiming = on != 0;
+  e.ksample = std::max(1, on);
+  e.kseq = 0;
   for (size_t i = 0; i < e.kcount.size(); i++) {
     e.kcount[i] = 0;
     e.kms[i] = 0.0;
     e.kwork[i] = 0.0;
     e.kbytes[i] = 0.0;
   }
+}
+
+int pqdts_device_timers(pqdts_ctx *ctx, int on) {
+  return guarded([&] {
+    Engine &e = E1(ctx);
+    HIPC(hipStreamSynchronize(e.s));
+    e.tresolve();
+    e.dev_timers = on != 0;
+  });
 }
 
 int pqdts_kernel_timing(pqdts_ctx *ctx, int on) {

@@ -8,7 +8,7 @@ lanes under a device-byte budget smaller than the stream's total footprint:
   pqdts_solve; mixed: a batch of one), whatever it was batched with;
 * the device bytes held by admitted problems never exceed the budget, requests
   waited in the queue for it, and every byte is returned at the end;
-* pqdts_footprint's estimate is what the context then holds.
+* pqdts_footprint's count is exactly what the context then holds.
 """
 import time
 
@@ -61,9 +61,9 @@ def test_footprint_is_what_the_context_holds(stream):
             else:
                 held = c.info()["device_bytes"]
             assert held == pers, (p.name, prec, held, pers)
-            dense = 8 * (p.D * p.M + p.M * p.M)  # dense F and G while the Gram matrix is formed
+            # the Gram band is formed in place: only a temporary descriptor on top
             if c.info()["gram"]:
-                assert dense <= trans < dense + 4096
+                assert 0 < trans < 4096
             else:
                 assert trans == 0
             c.close()

@@ -144,3 +144,26 @@ def test_start_stage2_and_maxiter_cap(cfg1):
     assert [int(v[0]) for v in its] == [2, 2, 2, 2]
     stages = [v for k, v in ev if k == pqdts_amd._lib.EV_STAGE]
     assert [int(v[0]) for v in stages] == [2]
+
+
+def test_device_timers_use_reference_names(cfg1):
+    """With device timers on, each section's time is the device time between events
+    recorded on the solver's stream: the reference's timer names appear
+    (pqdts.f90:103-186 and its tstart calls), sub-sections fit inside their
+    sections, and the whole minimisation fits inside the wall-clock time."""
+    import time
+    ctx = pqdts_amd.Context.from_problem(cfg1)
+    ctx.device_timers(True)
+    ctx.put("X", np.full((cfg1.M, cfg1.N), 1.0 / cfg1.N))
+    t0 = time.perf_counter()
+    ctx.solve(2, 200, 1e-6, 1, 0)
+    wall = time.perf_counter() - t0
+    t = ctx.timers()
+    for name in ("min", "L", "L_O", "dL", "dL_O", "dL_dLdX", "BF", "BF_O", "CG", "active_dLdX", "Lalpha_min",
+                 "doproj", "doproj2", "pdot"):
+        assert name in t and t[name][0] > 0 and t[name][1] > 0.0, name
+    assert t["min"][1] <= wall
+    assert t["BF_O"][1] <= t["BF"][1] * (1 + 1e-6)
+    assert t["dL_O"][1] + t["dL_dLdX"][1] <= t["dL"][1] * (1 + 1e-6)
+    assert t["CG"][1] <= t["min"][1] and t["Lalpha_min"][1] <= t["min"][1]
+    assert t["BF_O"][0] == t["BF"][0]  # one product per Hessian application on either path
