@@ -710,7 +710,9 @@ def workload_served(args, pqdts_amd, workloads):
         raise SystemExit("no GPU visible")
     budget = int(args.hbm_gb * 2**30)
     bounds = (20000, 400000)  # M x N of the lane boundaries
-    lane_cus = parse_lane_cus(args, pqdts_amd)
+    # the mix's lane layout ("auto") slows the large requests that bound this stream:
+    # every lane shares every CU unless ranges are given
+    lane_cus = None if args.lane_cus.strip() == "auto" else parse_lane_cus(args, pqdts_amd)
 
     def serve(precision):
         srv = pqdts_amd.Server(bounds, lane_cus=lane_cus, capacity=args.capacity or 256, budget_bytes=budget)

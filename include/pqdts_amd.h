@@ -174,12 +174,12 @@ int pqdts_kernel_timing(pqdts_ctx *ctx, int on);
 int pqdts_kernel_stats(pqdts_ctx *ctx, int cap, char (*names)[24], int *counts, double *ms, double *flops,
                        double *bytes);
 
-/* Sizes of a context: info[0..12] = M, N, D, stored F elements (sum of row span
+/* Sizes of a context: info[0..13] = M, N, D, stored F elements (sum of row span
  * lengths, the reference's Frows storage), packed elements of F's band tiles,
  * #F tiles, #F^T tiles, Gram path active (0/1), packed elements of G's band
  * tiles, stored elements of G's row bands, packed elements of F^T's band
  * tiles, device bytes held by the problem, split of the Gram product over band
- * chunks (1 = none; see pqdts_set_hvp_split). */
+ * chunks (1 = none; see pqdts_set_hvp_split), most band chunks of one G tile. */
 int pqdts_info(pqdts_ctx *ctx, int64_t *info);
 
 /* How BF (and every CG step) applies the Hessian: 1 = the reference's two

@@ -326,11 +326,11 @@ class Context:
         return conv.value, events
 
     def info(self) -> dict:
-        a = (C.c_int64 * 13)()
+        a = (C.c_int64 * 14)()
         _check(load().pqdts_info(self.h, a))
         return {"M": a[0], "N": a[1], "D": a[2], "nspan": a[3], "tile_elems": a[4], "row_tiles": a[5],
                 "col_tiles": a[6], "gram": bool(a[7]), "gram_tile_elems": a[8], "gram_nnz": a[9],
-                "ft_tile_elems": a[10], "device_bytes": a[11], "hvp_split": a[12]}
+                "ft_tile_elems": a[10], "device_bytes": a[11], "hvp_split": a[12], "gram_nch": a[13]}
 
     def set_hvp_path(self, path):
         """0 automatic, 1 the two F products, 2 the Gram matrix G = F^T F."""

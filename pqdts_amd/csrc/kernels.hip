@@ -37,8 +37,9 @@ constexpr int MAXB = 128;      // problems per launch
 // element kernels over the M x N arrays: blocks of BLOCK x ELEMS elements
 constexpr int ELEMS = 8;
 constexpr int EBLOCK = BLOCK * ELEMS;
-constexpr int NSLOT = 7;       // X G HP XTMP SN Z, then O as operand 6
+constexpr int NSLOT = 8;       // X G HP XTMP SN Z, then O and R = O - P as operands 6, 7
 constexpr int OP_O = 6;
+constexpr int OP_R = 7;
 
 // single-precision literals of the reference, widened (pqdts.f90:1207)
 constexpr double kActiveTol = (double)1e-8f;
@@ -56,7 +57,7 @@ struct Band {
 struct Prob {
   int M, N, D, pcap;
   double gamma;
-  double *op[NSLOT];     // X G HP XTMP SN Z O
+  double *op[NSLOT];     // X G HP XTMP SN Z O R
   double *P;
   const double *diag_full, *diag_red;
   int *maxr;
@@ -275,15 +276,17 @@ __device__ __forceinline__ double stencil_src(bool exp, const GLOBAL double *__r
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
-// B operand: as stored, with the implicit column expanded (reduced problems), minus
-// P, or the FP32 expanded copy of the CG direction (P.p32, written where the
-// direction is formed)
+// B operand: as stored, with the implicit column expanded (reduced problems), or
+// the FP32 expanded copy of the CG direction (P.p32, written where the direction
+// is formed)
 // B_SPAN: B is F itself (rows d, columns k) read from its span storage, zero
 // outside each row's span: the Gram matrix F^T F formed without a dense copy of F.
-enum BMode { B_PLAIN = 0, B_EXPAND = 1, B_DIFF = 2, B_P32 = 3, B_SPAN = 4 };
+enum BMode { B_PLAIN = 0, B_EXPAND = 1, B_P32 = 3, B_SPAN = 4 };
 // E_GBAND: the product is the Gram matrix, stored straight into G's band pack
 // (P.bg); output tiles outside the band of their row tile are not computed.
-enum EMode { E_STORE = 0, E_LOSS = 1, E_STORE_LOSS = 2, E_GRAD = 3, E_HVP = 4, E_PART = 5, E_GBAND = 6 };
+// E_STORE_R: store O = F X and R = O - P (the gradient's operand, formed once here
+// instead of in every row tile of the transposed product that reads it).
+enum EMode { E_STORE = 0, E_LOSS = 1, E_GRAD = 3, E_HVP = 4, E_PART = 5, E_GBAND = 6, E_STORE_R = 7 };
 enum BandSel { BAND_F = 0, BAND_FT = 1, BAND_G = 2 };
 
 struct GemmOps {
@@ -324,7 +327,7 @@ struct MfmaT<float> {
 //
 // All products of the solver are this kernel:
 //   O = F V        A = F   (D x M), B = V (M x N)              (redO, pqdts.f90:724-736)
-//   G = 2 F^T W    A = F^T (M x D), B = W = O - P or O (D x N)  (dL 998-1006, BF 280-288)
+//   G = 2 F^T W    A = F^T (M x D), B = W = R = O - P or O (D x N)  (dL 998-1006, BF 280-288)
 //   H p = 2 G p    A = G = F^T F (M x M), B = p (M x N)         (BF through the Gram matrix)
 // A is pre-packed per tile of TR rows as a dense, zero-filled, transposed band
 // [nch*KC][TR], so staging a chunk of A is one contiguous copy. B chunks (KC rows
@@ -396,7 +399,6 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
       } else if (k < nk && n < N) {
         const int64_t idx = (int64_t)k * N + n;
         v = Bp[idx];
-        if (BM == B_DIFF) v -= gp(P.P)[idx];
         if (BM == B_EXPAND && red) {
           // independent loads, selected afterwards
           const int mr = gp(P.maxr)[k];
@@ -470,7 +472,8 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
   constexpr int QB = 2;
   const double g2 = 2.0 * P.gamma;
   constexpr bool GH = (EM == E_GRAD || EM == E_HVP);
-  constexpr bool LOSS = (EM == E_LOSS || EM == E_STORE_LOSS);
+  constexpr bool LOSS = (EM == E_LOSS);
+  constexpr bool WITH_P = (EM == E_LOSS || EM == E_STORE_R);
   const bool exp = (EM == E_HVP) && red;
   const bool sten = GH && P.gamma != 0.0;
   const GLOBAL double *__restrict__ E = GH ? gp(P.op[ops.e]) : nullptr;
@@ -499,10 +502,8 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
           const int rp = rc + 1 < nrows ? rc + 1 : rc;
           const int rm = rc > 0 ? rc - 1 : rc;
           const int64_t idx = (int64_t)rc * N + nc;
-          if (LOSS) {
-            fh[u] = gp(P.fhas)[rc];
-            pv[u] = gp(P.P)[idx];
-          }
+          if (LOSS) fh[u] = gp(P.fhas)[rc];
+          if (WITH_P) pv[u] = gp(P.P)[idx];
           if (GH) {
             mr0[u] = red ? gp(P.maxr)[rc] : -1;
             gmv[u] = red ? gp(P.gm)[rc] : 0.0;
@@ -528,7 +529,8 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
           if (row[u] >= nrows || n >= N) continue;
           const int64_t idx = (int64_t)row[u] * N + n;
           const double v = FP32 ? acc64[i][j][q] : (double)acc[i][j][q];
-          if (EM == E_STORE || EM == E_STORE_LOSS || EM == E_PART) Cp[idx] = v;
+          if (EM == E_STORE || EM == E_STORE_R || EM == E_PART) Cp[idx] = v;
+          if (EM == E_STORE_R) gp(P.op[OP_R])[idx] = v - pv[u];
           if (LOSS && fh[u]) {
             const double t = v - pv[u];
             redv = fma(t, t, redv);
@@ -558,7 +560,8 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
           if (row >= nrows || n >= N) continue;
           const int64_t idx = (int64_t)row * N + n;
           const double v = FP32 ? acc64[i][j][q] : (double)acc[i][j][q];
-          if (EM == E_STORE || EM == E_STORE_LOSS || EM == E_PART) Cp[idx] = v;
+          if (EM == E_STORE || EM == E_STORE_R || EM == E_PART) Cp[idx] = v;
+          if (EM == E_STORE_R) gp(P.op[OP_R])[idx] = v - gp(P.P)[idx];
           if (EM == E_GBAND && n >= gk0 && n < gk0 + gkw)
             gp(const_cast<double *>(P.bg.pack))[gp(P.bg.off)[tile] + (int64_t)(n - gk0) * GT + (row - r0)] = v;
           if (LOSS && gp(P.fhas)[row]) {
@@ -631,7 +634,7 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
   }
   double redv = 0.0;
   band_gemm_tile<T, KC, BM, EM, MASK, DOT, LAT>(P, ops, lb, full, late, As, Bs, redv);
-  if (EM == E_LOSS || EM == E_STORE_LOSS || DOT) {
+  if (EM == E_LOSS || DOT) {
     const double b = block_sum(redv, sh);
     if (threadIdx.x == 0) gp(P.part)[(int64_t)ops.region * P.pcap + lb] = b;
     if (threadIdx.x == 0 && lb == 0) gp(P.pn)[ops.region] = ng;

@@ -125,7 +125,7 @@ struct pqdts_ctx {
   double *fval = nullptr;
   uint8_t *fhas = nullptr;
   int *kd_lo = nullptr, *kd_hi = nullptr;
-  double *P = nullptr, *O = nullptr;
+  double *P = nullptr, *O = nullptr, *R = nullptr;  // R = O - P, the gradient's operand
   double *slot[6] = {};
   double *diag_full = nullptr, *diag_red = nullptr;
   int *maxr = nullptr;
@@ -187,6 +187,7 @@ void refresh_desc(pqdts_ctx *c) {
   p.gamma = c->gamma;
   for (int i = 0; i < 6; i++) p.op[i] = c->slot[i];
   p.op[OP_O] = c->O;
+  p.op[OP_R] = c->R;
   p.P = c->P;
   p.diag_full = c->diag_full;
   p.diag_red = c->diag_red;
@@ -452,7 +453,9 @@ struct Engine {
   // that operand once (FP32 copy in mixed precision) + the dense operand read once
   // + the result written once (+ P for the loss and the gradient, the stencil
   // source for the gradient, the active-set mask for the Hessian product).
-  Work gemm_work(int p, int band, bool f32 = false) const {
+  // em: the epilogue (E_STORE_R also reads P and writes R; E_GRAD reads R and the
+  // stencil source X; E_HVP reads the direction for the stencil and the mask)
+  Work gemm_work(int p, int band, bool f32 = false, int em = E_HVP) const {
     const pqdts_ctx *c = ctx[p];
     const double MN = (double)c->MN, DN = (double)c->D * c->N, ab = f32 ? 4.0 : 8.0;
     Work w;
@@ -461,10 +464,10 @@ struct Engine {
       w.bytes = ab * (double)c->gram_nnz + 16.0 * MN + MN;
     } else if (band == BAND_F) {
       w.flops = 2.0 * (double)c->nspan * c->N;
-      w.bytes = ab * (double)c->nspan + 8.0 * MN + 8.0 * DN;
+      w.bytes = ab * (double)c->nspan + 8.0 * MN + (em == E_STORE_R ? 24.0 : 8.0) * DN;
     } else {
       w.flops = 2.0 * (double)c->nspan * c->N;
-      w.bytes = ab * (double)c->nspan + 16.0 * DN + 16.0 * MN + MN;
+      w.bytes = ab * (double)c->nspan + 8.0 * DN + 16.0 * MN + (em == E_GRAD ? 0.0 : MN);
     }
     return w;
   }
@@ -609,7 +612,7 @@ struct Engine {
     launch(S, [&](int p) { return nb_gemm(p, ops.band); }, [&](const WorkList &wl, int tot) {
       launch_gemm<T, BM, EM, MASK, DOT>(tot, s, dprobs, wl, ops);
     });
-    kend(e0, kname, S, [&](int p) { return gemm_work(p, ops.band, sizeof(T) == 4); });
+    kend(e0, kname, S, [&](int p) { return gemm_work(p, ops.band, sizeof(T) == 4, EM); });
   }
 
   // ================= operators (reference subroutine in brackets) =================
@@ -639,7 +642,7 @@ struct Engine {
       launch_gemm<double, B_PLAIN, E_LOSS, false, false>(tot, s, dprobs, wl, ops);
     });
     tstop("L_O");
-    kend(e0, "spmm_F", S, [&](int p) { return gemm_work(p, BAND_F); });
+    kend(e0, "spmm_F", S, [&](int p) { return gemm_work(p, BAND_F, false, E_LOSS); });
     launch(S, [](int) { return 1; }, [&](const WorkList &wl, int) {
       if (ixmin >= 0)
         kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 3, R_A, ia, R_B, ib, R_C, ixmin, 4);
@@ -654,14 +657,14 @@ struct Engine {
     struct Scope sc(this, "dL");
     reconstruct(S, x);
     tstart("dL_O");
-    gemm_t<double, B_PLAIN, E_STORE, false, false>(S, GemmOps{BAND_F, x, OP_O, 0, 0, 0}, "spmm_F");
+    gemm_t<double, B_PLAIN, E_STORE_R, false, false>(S, GemmOps{BAND_F, x, OP_O, 0, 0, 0}, "spmm_F");
     tstop("dL_O");
     tstart("dL_dLdX");
     std::vector<int> R = reduced_of(S);
     if (!R.empty())
       launch(R, [&](int p) { return nb_rows(p); },
              [&](const WorkList &wl, int tot) { kb_foldval<T_GRAD><<<tot, BLOCK, 0, s>>>(dprobs, wl, x); });
-    gemm_t<double, B_DIFF, E_GRAD, false, false>(S, GemmOps{BAND_FT, OP_O, g, x, 0, 0}, "spmmT_F");
+    gemm_t<double, B_PLAIN, E_GRAD, false, false>(S, GemmOps{BAND_FT, OP_R, g, x, 0, 0}, "spmmT_F");
     tstop("dL_dLdX");
   }
 
@@ -1342,9 +1345,29 @@ void ensure_gram(pqdts_ctx *c) {
 // more than the shorter chain saves. An explicit split applies to both. The
 // split depends on the problem and its precision alone, so a problem computes
 // the same bits alone and in a batch.
+// The table TUNED_SPLIT (written by the offline autotuner, pqdts_amd/autotune.py,
+// and compiled in) gives the fastest measured split per bucket of
+// (log2 output tiles, log2 most band chunks of a tile); 0 marks a bucket no tuned
+// shape reached, which falls back to the rule: the largest k in {2, 4} with
+// k x tiles <= 1024 and at least 6 chunks per slice of the longest tile.
+#include "tuned_split.inc"
+
+int ilog2_floor(int64_t v) {
+  int b = 0;
+  while (v > 1) {
+    v >>= 1;
+    b++;
+  }
+  return b;
+}
+
 int split_by_shape(int64_t N, int gram_tiles, int gram_nch) {
-  int hs = 1;
   const int64_t ntile = (int64_t)cdiv(N, TILE) * gram_tiles;
+  const int b1 = std::min(TUNED_TILE_BUCKETS - 1, ilog2_floor(std::max<int64_t>(ntile, 1)));
+  const int b2 = std::min(TUNED_CHUNK_BUCKETS - 1, ilog2_floor(std::max(gram_nch, 1)));
+  const int tuned = TUNED_SPLIT[b1][b2];
+  if (tuned > 0) return std::min(tuned, std::max(gram_nch, 1));
+  int hs = 1;
   for (int k = 2; k <= 4; k *= 2)
     if (k * ntile <= 1024 && gram_nch >= 6 * k) hs = k;
   return hs;
@@ -1389,7 +1412,7 @@ void destroy_ctx(pqdts_ctx *c) {
   free_band(c->bt);
   free_band(c->bg);
   void *ptrs[] = {c->flo,       c->flen,     c->foff, c->fval, c->fhas, c->kd_lo, c->kd_hi, c->P,
-                  c->O,         c->diag_full, c->diag_red, c->maxr, c->ik,  c->ev,    c->gm,    c->lam,
+                  c->O,         c->R,         c->diag_full, c->diag_red, c->maxr, c->ik,  c->ev,    c->gm,    c->lam,
                   c->lmin,      c->part,     c->scal, c->ddesc, c->pn, c->hw, c->p32, c->pb};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1510,7 +1533,7 @@ int64_t footprint_bytes(int64_t M, int64_t N, int64_t D, int64_t nspan, int64_t 
   const int64_t MN = M * N, DN = D * N;
   const int64_t pcap = std::max<int64_t>({(int64_t)cdiv(N, TILE) * std::max(bf_tiles, bt_tiles),
                                           cdiv(M, WAVES_PER_BLOCK), cdiv(MN, EBLOCK), 1});
-  int64_t b = D * (4 + 4 + 8 + 1) + 8 * nspan + M * (4 + 4) + 16 * DN + 48 * MN + M * (8 + 8 + 4) + MN +
+  int64_t b = D * (4 + 4 + 8 + 1) + 8 * nspan + M * (4 + 4) + 24 * DN + 48 * MN + M * (8 + 8 + 4) + MN +
               M * 4 * 8 + 8 * pcap * NREG + 8 * NSCAL + 4 * NREG + (int64_t)sizeof(Prob);
   // a band pack holds at least one element
   auto band = [&](int64_t elems, int tiles) { return 8 * std::max<int64_t>(elems, 1) + 16 * (int64_t)tiles; };
@@ -1598,6 +1621,7 @@ pqdts_ctx *create_ctx(int64_t M, int64_t N, int64_t D, int64_t nF, const int32_t
     c->kd_hi = dalloc<int>(M);
     c->P = dalloc<double>(c->DN);
     c->O = dalloc<double>(c->DN);
+    c->R = dalloc<double>(c->DN);
     for (int s = 0; s < 6; s++) c->slot[s] = dalloc<double>(c->MN);
     c->diag_full = dalloc<double>(M);
     c->diag_red = dalloc<double>(M);
@@ -2140,6 +2164,7 @@ int pqdts_info(pqdts_ctx *ctx, int64_t *info) {
     // device bytes held by this problem
     info[11] = ctx_device_bytes(ctx);
     info[12] = ctx->hs;
+    info[13] = ctx->gram_nch;
   });
 }
 

@@ -73,3 +73,15 @@ def test_cu_mask_must_be_one_consecutive_range():
     words, n = _lib.cu_mask_words(range(32, 64), 256)
     assert L.pqdts_batch_set_cu_mask(None, words, n) != 0
     assert b"null batch" in L.pqdts_last_error()
+
+
+def test_tuned_split_table_is_well_formed():
+    """The autotuner's committed table (read by the library's split rule): 11 x 10
+    buckets, each 0 (no tuned shape: the rule) or a split of 1, 2 or 4."""
+    txt = open(os.path.join(ROOT, "pqdts_amd", "csrc", "tuned_split.inc")).read()
+    body = txt[txt.index("= {") + 3: txt.rindex("};")]
+    rows = re.findall(r"\{([^}]*)\}", body)
+    assert len(rows) == 11
+    for r in rows:
+        vals = [int(v) for v in r.split(",")]
+        assert len(vals) == 10 and set(vals) <= {0, 1, 2, 4}
