@@ -244,6 +244,7 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, timing, tag, lane
         batches.append(b)
     lat, res = [None] * len(problems), [None] * len(problems)
     kst, errors = {}, []
+    kst_lane = [None] * len(groups)
     lock = threading.Lock()
     t0 = time.perf_counter()
 
@@ -264,8 +265,10 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, timing, tag, lane
                         f"{time.perf_counter() - t0:.1f} s")
             log(f"  {tag} lane {g}: {len(idx)} problems done in {steps} steps, {time.perf_counter() - t0:.1f} s")
             if timing:
+                mine = b.kernel_stats()
+                kst_lane[g] = mine
                 with lock:
-                    for name, v in b.kernel_stats().items():
+                    for name, v in mine.items():
                         old = kst.get(name, (0, 0.0, 0.0, 0.0))
                         kst[name] = tuple(x + y for x, y in zip(old, v))
         except Exception as e:  # reported by the caller
@@ -282,7 +285,29 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, timing, tag, lane
     if errors:
         raise errors[0]
     X = [c.get("X") for c in ctxs]
+    if timing:
+        kst["__lanes__"] = kst_lane
     return dt, lat, X, res, kst
+
+
+def lane_rooflines(kst):
+    """The dominant band product of each lane (smallest problems first): launches, average
+    duration, achieved TFLOP/s, fraction of its MFMA peak."""
+    out = []
+    for ks in kst.get("__lanes__") or []:
+        if not ks:
+            out.append(None)
+            continue
+        prods = {k: v for k, v in ks.items() if v[2] > 0}
+        if not prods:
+            out.append(None)
+            continue
+        k = max(prods, key=lambda n: prods[n][1])
+        n, ms, fl, _ = prods[k]
+        peak = FP32_PEAK if k.endswith("32") else FP64_PEAK
+        tf = fl / (ms / 1e3) / 1e12 if ms else 0.0
+        out.append({"kernel": k, "launches": n, "avg_us": 1e3 * ms / max(n, 1), "tflops": tf, "frac": tf / peak})
+    return out
 
 
 def objective(ctx):
@@ -422,9 +447,12 @@ def workload_mix(args, pqdts_amd, workloads):
     for s in range(args.steps):
         dt, lat, Xm, res, kst = run_batched(pqdts_amd, problems, ctxs, 1, args.maxiter, args.ksample,
                                             f"batched mixed step {s}", args.lanes, lane_cus)
+        lanes_kst = kst.pop("__lanes__", None)
         for name, v in kst.items():
             old = kst_m.get(name, (0, 0.0, 0.0, 0.0))
             kst_m[name] = tuple(x + y for x, y in zip(old, v))
+        if s == 0:
+            lanes_rf = lane_rooflines({"__lanes__": lanes_kst})
         if runs and not all(np.array_equal(a, b) for a, b in zip(runs[0][2], Xm)):
             raise SystemExit("batched mixed results differ between steps")
         runs.append((dt, lat, Xm if not runs else runs[0][2], res))
@@ -438,6 +466,8 @@ def workload_mix(args, pqdts_amd, workloads):
     if args.legs == "timed":
         # only the timed region (for profiler runs): the line carries the timing and the roofline
         rf = roofline_of(kst_m)
+        if rf is not None:
+            rf["per_lane"] = lanes_rf
         return {"metric": METRIC, "value": None, "unit": "solves/s", "n_gpus": 1, "steps": steps,
                 "warmup": args.warmup, "ms_per_step": 1e3 * total / steps, "higher_is_better": True,
                 "scaling": "weak", "vs_baseline": None, "dtype": "f32 Hessian products / f64 elsewhere",
@@ -501,6 +531,7 @@ def workload_mix(args, pqdts_amd, workloads):
     seq_lat = np.cumsum([seq_t[i] for i in seq_idx]) * args.seq_stride
     rf = roofline_of(kst_m)
     if rf is not None:
+        rf["per_lane"] = lanes_rf
         tr = measured_traffic(rf["kernel"])
         rf["traffic"] = tr
         rf["traffic_source"] = os.path.relpath(PMC_SUMMARY, ROOT) if tr is not None else None

@@ -85,3 +85,18 @@ def test_tuned_split_table_is_well_formed():
     for r in rows:
         vals = [int(v) for v in r.split(",")]
         assert len(vals) == 10 and set(vals) <= {0, 1, 2, 4}
+
+
+def test_footprint_runs_on_the_host():
+    """pqdts_footprint needs no GPU: it plans the context from F's pattern. Mixed
+    precision adds the FP32 band copies and the FP32 direction copy (more than 4 M N
+    bytes); a problem on the Gram path needs only a descriptor on top while its band
+    is formed; a wider Pi costs at least its six work arrays per extra column."""
+    prob = workloads.config1()
+    p64, t64 = pqdts_amd.footprint(prob, 0)
+    p32, t32 = pqdts_amd.footprint(prob, 1)
+    assert p64 > 8 * 6 * prob.M * prob.N and p32 > p64 + 4 * prob.M * prob.N
+    assert 0 <= t64 < 4096 and t64 == t32
+    import dataclasses
+    wide = dataclasses.replace(prob, N=2 * prob.N, P=np.zeros((prob.D, 2 * prob.N)))
+    assert pqdts_amd.footprint(wide, 0)[0] >= p64 + 48 * prob.M * prob.N
