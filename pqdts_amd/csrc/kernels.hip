@@ -462,13 +462,14 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
     __syncthreads();
   }
 
-  // Epilogue. LAT (latency form, FP64, for launches too small to fill the chip):
-  // in batches of QB outputs (q) of one MFMA tile (i, j) per lane, every operand of
-  // a batch is loaded first (indices clamped into range, values of out-of-range
+  // Epilogue. LAT (latency form, for launches too small to fill the chip): in
+  // batches of QB outputs (q) of one MFMA tile (i, j) per lane, every operand of a
+  // batch is loaded first (indices clamped into range, values of out-of-range
   // outputs discarded) so the loads overlap instead of each waiting on the
   // previous output's store; then the outputs are formed in (i, j, q) order.
   // Otherwise one output at a time (fewer registers: more waves per SIMD, which
-  // hides the same latency when the launch fills the chip). Same arithmetic.
+  // hides the same latency when the launch fills the chip). Same arithmetic, in
+  // the same order, so the two forms give the same bits.
   constexpr int QB = 2;
   const double g2 = 2.0 * P.gamma;
   constexpr bool GH = (EM == E_GRAD || EM == E_HVP);
@@ -480,7 +481,10 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
   const GLOBAL double *__restrict__ dd = DOT ? gp(P.op[ops.dd]) : nullptr;
   // the late fold exists for the masked FP32 Hessian product with its dot (CG)
   constexpr bool LATE_OK = FP32 && EM == E_HVP && MASK && DOT;
-  if constexpr (!FP32 && LAT) {
+  // the late fold writes gm (the row's fold value) instead of reading it
+  const bool lfold = LATE_OK && late && red;
+  static_assert(!(LAT && EM == E_GBAND), "the Gram band is formed in the occupancy form only");
+  if constexpr (LAT) {
 #pragma unroll
   for (int i = 0; i < MR; i++)
 #pragma unroll
@@ -493,7 +497,7 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
         const int n = n0 + wc0 + j * 16 + fi;
         const int nc = n < N ? n : N - 1;
         int row[QB], mr0[QB];
-        double s0[QB], sp[QB], sm[QB], gmv[QB], pv[QB], dv[QB];
+        double s0[QB], sp[QB], sm[QB], gmv[QB], pv[QB], dv[QB], pbv[QB];
         uint8_t ikv[QB], fh[QB];
 #pragma unroll
         for (int u = 0; u < QB; u++) {
@@ -506,7 +510,8 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
           if (WITH_P) pv[u] = gp(P.P)[idx];
           if (GH) {
             mr0[u] = red ? gp(P.maxr)[rc] : -1;
-            gmv[u] = red ? gp(P.gm)[rc] : 0.0;
+            gmv[u] = red && !lfold ? gp(P.gm)[rc] : 0.0;
+            if (LATE_OK) pbv[u] = lfold ? gp(P.pb)[rc] : 0.0;
             if (sten) {
               s0[u] = E[idx];
               sp[u] = E[(int64_t)rp * N + nc];
@@ -541,10 +546,22 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
               if (row[u] + 1 < nrows) g = g + g2 * (s0[u] - sp[u]);
               if (row[u] > 0) g = g - g2 * (sm[u] - s0[u]);
             }
-            if (red) g = (n == mr0[u]) ? 0.0 : g - gmv[u];
-            if (MASK && ikv[u]) g = 0.0;
-            Cp[idx] = g;
-            if (DOT) redv = fma(dv[u], g, redv);
+            if (LATE_OK && lfold) {
+              // the late fold, as in the occupancy form below
+              const bool m = ikv[u] != 0;
+              if (n == mr0[u]) {
+                gp(P.gm)[row[u]] = g;
+                redv = fma(-g, pbv[u], redv);
+              } else if (!m) {
+                redv = fma(dv[u], g, redv);
+              }
+              Cp[idx] = m ? 0.0 : g;
+            } else {
+              if (red) g = (n == mr0[u]) ? 0.0 : g - gmv[u];
+              if (MASK && ikv[u]) g = 0.0;
+              Cp[idx] = g;
+              if (DOT) redv = fma(dv[u], g, redv);
+            }
           }
         }
       }

@@ -77,18 +77,16 @@ enum { R_A = 0, R_B = 1, R_C = 2 };
 
 inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
-// Band GEMM launch. FP64 launches with fewer workgroups than fill the chip (4 per
-// CU) take the latency form of the epilogue, larger ones the occupancy form (the
-// two measured 4-13 % apart per CG round, in opposite directions, in the two
-// regimes). FP32 has one form.
+// Band GEMM launch. Launches with fewer workgroups than fill the chip (4 per CU)
+// take the latency form of the epilogue, larger ones the occupancy form (for FP64
+// the two measured 4-13 % apart per CG round, in opposite directions, in the two
+// regimes). Both forms compute the same bits.
 constexpr int LAT_BLOCKS = 1024;
 template <typename T, int BM, int EM, bool MASK, bool DOT>
 void launch_gemm(int tot, hipStream_t s, const Prob *probs, const WorkList &wl, const GemmOps &ops) {
-  if constexpr (sizeof(T) == 8) {
-    if (tot < LAT_BLOCKS) {
-      kb_band_gemm<T, KC, BM, EM, MASK, DOT, true><<<tot, BLOCK, 0, s>>>(probs, wl, ops);
-      return;
-    }
+  if (tot < LAT_BLOCKS) {
+    kb_band_gemm<T, KC, BM, EM, MASK, DOT, true><<<tot, BLOCK, 0, s>>>(probs, wl, ops);
+    return;
   }
   kb_band_gemm<T, KC, BM, EM, MASK, DOT, false><<<tot, BLOCK, 0, s>>>(probs, wl, ops);
 }
