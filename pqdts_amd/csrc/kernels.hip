@@ -340,19 +340,107 @@ struct MfmaT<float> {
 // ---------------------------------------------------------------------------
 //
 // band_gemm_tile computes output tile lb (64 x 64) of one problem with the whole
-// workgroup, LDS buffers As/Bs supplied by the caller, and adds the tile's loss or
+// workgroup, LDS buffers supplied by the caller, and adds the tile's loss or
 // v2 contributions to redv; kb_band_gemm runs one tile per workgroup over a work
 // list.
 // ---------------------------------------------------------------------------
 constexpr int GT = 64;  // rows and columns of an output tile
+// Staging of band chunks in LDS. Two forms:
+//  * registers (every operand but the FP32 direction): each thread loads its share
+//    of chunk c+1 into registers while chunk c feeds the MFMAs, then stores it into
+//    the other of two LDS buffers. Row stride lds_stride(): the MFMA operand reads of
+//    a wave take 16 consecutive elements from each of 4 consecutive rows, and FP32
+//    rows padded to 80 elements put those 4 groups on disjoint LDS banks (FP64 reads,
+//    8 bytes a lane, are conflict-free unpadded).
+//  * direct (FP32 products whose B is the FP32 CG direction, B_P32): global loads
+//    write LDS without passing through registers, into a ring of DMA_STAGES buffers,
+//    so DMA_STAGES - 1 chunks are in flight while one is multiplied. Rows are 64
+//    elements, row r rotated by 16 (r mod 4) elements for the same bank spread.
+constexpr int DMA_STAGES = 4;
+template <typename T>
+constexpr int lds_stride() {
+  return sizeof(T) == 4 ? GT + 16 : GT;
+}
+template <typename T, int BM>
+constexpr bool gemm_dma() {
+  return sizeof(T) == 4 && BM == B_P32;
+}
+// buffers per operand, and elements per buffer
+template <typename T, int BM>
+constexpr int gemm_nbuf() {
+  return gemm_dma<T, BM>() ? DMA_STAGES : 2;
+}
+template <typename T, int BM, int KC>
+constexpr int gemm_lsz() {
+  return gemm_dma<T, BM>() ? KC * GT : KC * lds_stride<T>();
+}
+// s_waitcnt immediate (gfx9 encoding) that waits until at most n vector memory
+// operations of the wave are outstanding and does not wait on the other counters
+constexpr int vmcnt_le(int n) {
+  return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14);
+}
+#define LDS_AS __attribute__((address_space(3)))
 
+// Direct form: wave w copies rows 4w .. 4w+3 of a chunk, A as one 16-byte piece a
+// lane (asrc: this lane's piece), B one element a lane and one row an operation (5
+// vector memory operations a wave in all). B is read at indices clamped into the
+// direction (k < nk, n < N): where the clamp applies, the row's A is zero (the band
+// pack is zero-filled past the band) or the column is outside the output, so the
+// products are those of a zero-filled B. Row r of a buffer holds element x at
+// (x + 16 (r mod 4)) mod 64.
+constexpr int DMA_OPS = 5;
+template <int KC>
+__device__ __forceinline__ void dma_issue(float *wa, float *wb, const GLOBAL float *asrc, const GLOBAL float *p32,
+                                          int kb, int nk, int N, int nb, int w, int lane) {
+  __builtin_amdgcn_global_load_lds((const GLOBAL void *)asrc, (LDS_AS void *)(wa + w * 4 * GT), 16, 0, 0);
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int kk = 4 * w + u;
+    const int k = kb + kk, n = nb + ((lane - 16 * u) & (GT - 1));
+    const int kc = k < nk ? k : nk - 1, nc = n < N ? n : N - 1;
+    __builtin_amdgcn_global_load_lds((const GLOBAL void *)(p32 + (int64_t)kc * N + nc),
+                                     (LDS_AS void *)(wb + kk * GT), 4, 0, 0);
+  }
+}
+// One chunk of the direct form: copies a later chunk (when `issue`) into wa/wb and
+// multiplies the chunk in ra/rb into acc. Every pointer the copies and reads use is a
+// restrict-qualified parameter, so that the compiler knows the copies do not write
+// what is read here (it would otherwise wait for every copy in flight before the
+// first read).
+template <int KC>
+__device__ __forceinline__ void dma_chunk(const float *__restrict__ ra, const float *__restrict__ rb,
+                                          float *__restrict__ wa, float *__restrict__ wb, bool issue,
+                                          const GLOBAL float *__restrict__ asrc, const GLOBAL float *__restrict__ p32,
+                                          int kb, int nk, int N, int nb, int w, int lane, int wr0, int wc0,
+                                          f4_t (&acc)[2][2]) {
+  if (issue) dma_issue<KC>(wa, wb, asrc, p32, kb, nk, N, nb, w, lane);
+  const int fk = lane >> 4, fi = lane & 15;
+#pragma unroll
+  for (int ks = 0; ks < KC; ks += 4) {
+    const int r = ks + fk;  // r mod 4 = fk
+    float af[2], bf[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) af[i] = ra[r * GT + ((wr0 + i * 16 + fi + 16 * fk) & (GT - 1))];
+#pragma unroll
+    for (int j = 0; j < 2; j++) bf[j] = rb[r * GT + ((wc0 + j * 16 + fi + 16 * fk) & (GT - 1))];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = MfmaT<float>::mfma(af[i], bf[j], acc[i][j]);
+  }
+}
+
+// lds: A buffers then B buffers, gemm_nbuf() of gemm_lsz() elements each
 template <typename T, int KC, int BM, int EM, bool MASK, bool DOT, bool LAT>
-__device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops, int lb, bool full, bool late,
-                                               T (*As)[KC * GT], T (*Bs)[KC * GT], double &redv) {
+__device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops, int lb, bool full, bool late, T *lds,
+                                               double &redv) {
   constexpr int TR = GT, TC = GT, WC = 2, MR = 2, MC = 2;
   constexpr int AE = KC * TR / BLOCK;
   constexpr int BE = KC * TC / BLOCK;
+  constexpr int LS = lds_stride<T>();
   constexpr bool FP32 = sizeof(T) == 4;
+  // FP32: each thread stages 4 consecutive elements of A (one 16-byte load)
+  static_assert(!FP32 || AE == 4, "FP32 A staging takes one float4 per thread");
   typedef MfmaT<T> Mf;
   const bool red = !full;
   const Band &band = ops.band == BAND_F ? P.bf : (ops.band == BAND_FT ? P.bt : P.bg);
@@ -380,20 +468,32 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
   const int gkw = EM == E_GBAND ? gp(P.bg.nch)[tile] * KC : 0;
   if (EM == E_GBAND && (n0 + TC <= gk0 || n0 >= gk0 + gkw)) return;  // uniform over the block
 
-  T ra[AE], rb[BE];
+  constexpr bool DMA = gemm_dma<T, BM>();
+  constexpr int NBUF = gemm_nbuf<T, BM>(), LSZ = gemm_lsz<T, BM, KC>();
+  static_assert(!DMA || (KC == 16 && BLOCK == 256 && DMA_STAGES == 4),
+                "the direct staging maps 4 waves onto 16-row chunks and a 4-stage ring");
+  T *const As = lds;               // As + b * LSZ: A buffer b
+  T *const Bs = lds + NBUF * LSZ;  // Bs + b * LSZ: B buffer b
+
+  static_assert(BM != B_P32 || FP32, "the FP32 direction copy feeds FP32 products only");
+  T ra[AE], rb[BE];  // register form only
   auto gload = [&](int c) {
     const GLOBAL T *src = ap + (int64_t)c * (KC * TR);
+    if constexpr (FP32) {
+      const f4_t a4 = *(const GLOBAL f4_t *)(src + 4 * tid);
 #pragma unroll
-    for (int i = 0; i < AE; i++) ra[i] = src[tid + BLOCK * i];
+      for (int i = 0; i < AE; i++) ra[i] = a4[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < AE; i++) ra[i] = src[tid + BLOCK * i];
+    }
 #pragma unroll
     for (int i = 0; i < BE; i++) {
       const int e = tid + BLOCK * i;
       const int kk = e / TC, cc = e - kk * TC;
       const int k = k0 + c * KC + kk, n = n0 + cc;
       double v = 0.0;
-      if (BM == B_P32 && k < nk && n < N) {
-        v = (double)gp(P.p32)[(int64_t)k * N + n];
-      } else if (BM == B_SPAN && k < nk && n < N) {
+      if (BM == B_SPAN && k < nk && n < N) {
         const int rel = n - gp(P.flo)[k];
         if (rel >= 0 && rel < gp(P.flen)[k]) v = gp(P.fval)[gp(P.foff)[k] + rel];
       } else if (k < nk && n < N) {
@@ -410,11 +510,24 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
     }
   };
   auto sstore = [&](int buf) {
+    T *a = As + buf * LSZ, *b = Bs + buf * LSZ;
+    if constexpr (FP32) {
+      // thread t holds elements 4t .. 4t+3 of the chunk: row t/16, columns 4 (t mod 16) ..
 #pragma unroll
-    for (int i = 0; i < AE; i++) As[buf][tid + BLOCK * i] = ra[i];
+      for (int i = 0; i < AE; i++) a[(tid >> 4) * LS + 4 * (tid & 15) + i] = ra[i];
+    } else {
 #pragma unroll
-    for (int i = 0; i < BE; i++) Bs[buf][tid + BLOCK * i] = rb[i];
+      for (int i = 0; i < AE; i++) a[tid + BLOCK * i] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < BE; i++) {
+      const int e = tid + BLOCK * i;
+      b[(e / TC) * LS + (e % TC)] = rb[i];
+    }
   };
+  const GLOBAL float *p32 = gp(P.p32);
+  // this lane's 16-byte piece of A within a chunk (row 4w + lane/16, rotated)
+  const int a_piece = (4 * w + (lane >> 4)) * TR + ((4 * (lane & 15) - 16 * (lane >> 4)) & (GT - 1));
 
   typename Mf::acc_t acc[MR][MC];
   d4_t acc64[MR][MC];  // FP32 path: chunk sums accumulated in FP64
@@ -426,26 +539,55 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
       acc64[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
     }
 
-  if (nch > 0) {
-    gload(0);
-    sstore(0);
+  if constexpr (DMA) {
+#pragma unroll
+    for (int c = 0; c < DMA_STAGES - 1; c++)
+      if (c < nch)
+        dma_issue<KC>(As + c * LSZ, Bs + c * LSZ, (const GLOBAL float *)ap + (int64_t)c * (KC * TR) + a_piece, p32,
+                      k0 + c * KC, nk, N, n0, w, lane);
+  } else {
+    if (nch > 0) {
+      gload(0);
+      sstore(0);
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int fk = lane >> 4, fi = lane & 15;
   for (int c = 0; c < nch; c++) {
-    const int buf = c & 1;
-    if (c + 1 < nch) gload(c + 1);
+    if constexpr (DMA) {
+      // this wave's copies of chunk c have landed once at most the copies of the
+      // chunks issued after it are outstanding; the barrier then makes the whole
+      // chunk visible and tells every wave that buffer (c - 1) mod 4 has been read,
+      // so chunk c + 3 may overwrite it
+      const int later = nch - 1 - c;
+      if (later >= DMA_STAGES - 2)
+        __builtin_amdgcn_s_waitcnt(vmcnt_le((DMA_STAGES - 2) * DMA_OPS));
+      else if (later == 1)
+        __builtin_amdgcn_s_waitcnt(vmcnt_le(DMA_OPS));
+      else
+        __builtin_amdgcn_s_waitcnt(vmcnt_le(0));
+      __builtin_amdgcn_s_barrier();
+      const int st = c & (DMA_STAGES - 1), wt = (c + DMA_STAGES - 1) & (DMA_STAGES - 1);
+      const int cn = c + DMA_STAGES - 1;
+      dma_chunk<KC>(As + st * LSZ, Bs + st * LSZ, As + wt * LSZ, Bs + wt * LSZ, cn < nch,
+                      (const GLOBAL float *)ap + (int64_t)cn * (KC * TR) + a_piece, p32, k0 + cn * KC, nk, N, n0, w,
+                      lane, wr0, wc0, acc);
+    } else {
+      const int buf = c & 1;
+      if (c + 1 < nch) gload(c + 1);
+      const T *a = As + buf * LSZ, *b = Bs + buf * LSZ;
 #pragma unroll
-    for (int ks = 0; ks < KC; ks += 4) {
-      T af[MR], bf[MC];
+      for (int ks = 0; ks < KC; ks += 4) {
+        T af[MR], bf[MC];
 #pragma unroll
-      for (int i = 0; i < MR; i++) af[i] = As[buf][(ks + fk) * TR + wr0 + i * 16 + fi];
+        for (int i = 0; i < MR; i++) af[i] = a[(ks + fk) * LS + wr0 + i * 16 + fi];
 #pragma unroll
-      for (int j = 0; j < MC; j++) bf[j] = Bs[buf][(ks + fk) * TC + wc0 + j * 16 + fi];
+        for (int j = 0; j < MC; j++) bf[j] = b[(ks + fk) * LS + wc0 + j * 16 + fi];
 #pragma unroll
-      for (int i = 0; i < MR; i++)
+        for (int i = 0; i < MR; i++)
 #pragma unroll
-        for (int j = 0; j < MC; j++) acc[i][j] = Mf::mfma(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < MC; j++) acc[i][j] = Mf::mfma(af[i], bf[j], acc[i][j]);
+      }
     }
     if (FP32) {
 #pragma unroll
@@ -458,8 +600,10 @@ __device__ __forceinline__ void band_gemm_tile(const Prob &P, const GemmOps &ops
             acc[i][j][q] = 0;
           }
     }
-    if (c + 1 < nch) sstore(buf ^ 1);
-    __syncthreads();
+    if constexpr (!DMA) {
+      if (c + 1 < nch) sstore((c & 1) ^ 1);
+      __syncthreads();
+    }
   }
 
   // Epilogue. LAT (latency form, for launches too small to fill the chip): in
@@ -637,9 +781,14 @@ __device__ __forceinline__ void smooth_part(const Prob &P, int x, int lb, int nb
 // the smoothing partials of X: one launch for both terms of the objective.
 template <typename T, int KC, int BM, int EM, bool MASK, bool DOT, bool LAT>
 __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ probs, WorkList wl, GemmOps ops) {
-  __shared__ T As[2][KC * GT];
-  __shared__ T Bs[2][KC * GT];
-  __shared__ double sh[WAVES_PER_BLOCK];
+  // One LDS array for the chunk buffers and the block-sum scratch, which overlaps the
+  // first buffer and is used once the product is done (after a barrier). With a second
+  // LDS variable in the kernel the compiler tags every LDS access with the variable it
+  // belongs to and loses the distinction between the ring buffers of the direct form,
+  // whose reads it would then make wait for every copy in flight; and 32 KB exactly
+  // leaves room for 5 workgroups a CU.
+  __shared__ __align__(16) T lds[2 * gemm_nbuf<T, BM>() * gemm_lsz<T, BM, KC>()];
+  double *const sh = reinterpret_cast<double *>(lds);
   BATCH_PROLOGUE();
   int ng = nblk;  // blocks of the product itself
   if (EM == E_LOSS && ops.sm >= 0) {
@@ -650,8 +799,9 @@ __global__ __launch_bounds__(BLOCK) void kb_band_gemm(const Prob *__restrict__ p
     }
   }
   double redv = 0.0;
-  band_gemm_tile<T, KC, BM, EM, MASK, DOT, LAT>(P, ops, lb, full, late, As, Bs, redv);
+  band_gemm_tile<T, KC, BM, EM, MASK, DOT, LAT>(P, ops, lb, full, late, lds, redv);
   if (EM == E_LOSS || DOT) {
+    __syncthreads();  // every wave is done with the chunk buffers that sh overlaps
     const double b = block_sum(redv, sh);
     if (threadIdx.x == 0) gp(P.part)[(int64_t)ops.region * P.pcap + lb] = b;
     if (threadIdx.x == 0 && lb == 0) gp(P.pn)[ops.region] = ng;
