@@ -1367,6 +1367,44 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
 // direction update forms the same quotient from r.
 // With `lf` (F_LATE in reduced space) Hp is folded here: 0 at the implicit column
 // and on the active set, Hp - gm[j] elsewhere.
+// The row's arrays are restrict-qualified parameters (they are distinct arrays), and
+// each lane loads the operands of ROW_U elements before it stores any, so that the
+// loads overlap instead of each waiting on the previous element's stores.
+constexpr int ROW_U = 4;  // elements a lane loads before it computes and stores them
+template <bool LF>
+__device__ __forceinline__ void cg_update_span(GLOBAL double *__restrict__ sn, const GLOBAL double *__restrict__ p,
+                                               const GLOBAL double *__restrict__ hp, GLOBAL double *__restrict__ r,
+                                               const GLOBAL uint8_t *__restrict__ ik, int N, int mr, double gmj,
+                                               double sq, double a, double &rz, double &rr) {
+  for (int n0 = lane_id(); n0 < N; n0 += ROW_U * WAVE) {
+    // the operands of ROW_U elements first (indices clamped into the row), so that
+    // their loads overlap; then the elements in increasing n
+    double sv[ROW_U], pv[ROW_U], hv[ROW_U], rv[ROW_U];
+    uint8_t kv[ROW_U];
+#pragma unroll
+    for (int u = 0; u < ROW_U; u++) {
+      const int n = n0 + u * WAVE, nc = n < N ? n : N - 1;
+      sv[u] = sn[nc];
+      pv[u] = p[nc];
+      hv[u] = hp[nc];
+      rv[u] = r[nc];
+      kv[u] = LF ? ik[nc] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < ROW_U; u++) {
+      const int n = n0 + u * WAVE;
+      if (n >= N) break;
+      sn[n] = sv[u] + a * pv[u];
+      double h = hv[u];
+      if (LF) h = (n == mr || kv[u]) ? 0.0 : h - gmj;
+      const double x = rv[u] + (-a) * h;
+      r[n] = x;
+      const double zz = x / sq;
+      rz = fma(x, zz, rz);
+      rr = fma(x, x, rr);
+    }
+  }
+}
 __device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, bool lf, double a, double &rz,
                                               double &rr) {
   const int N = P.N;
@@ -1374,22 +1412,49 @@ __device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, b
   const int mr = lf ? gp(P.maxr)[j] : -1;
   const double gmj = lf ? gp(P.gm)[j] : 0.0;
   const int64_t base = (int64_t)j * N;
-  for (int n = lane_id(); n < N; n += WAVE) {
-    const int64_t idx = base + n;
-    gp(P.op[4])[idx] = gp(P.op[4])[idx] + a * gp(P.op[3])[idx];
-    double hp = gp(P.op[2])[idx];
-    if (lf) hp = (n == mr || gp(P.ik)[idx]) ? 0.0 : hp - gmj;
-    const double r = gp(P.op[1])[idx] + (-a) * hp;
-    gp(P.op[1])[idx] = r;
-    const double zz = r / sq;
-    rz = fma(r, zz, rz);
-    rr = fma(r, r, rr);
-  }
+  if (lf)
+    cg_update_span<true>(gp(P.op[4]) + base, gp(P.op[3]) + base, gp(P.op[2]) + base, gp(P.op[1]) + base,
+                         gp(P.ik) + base, N, mr, gmj, sq, a, rz, rr);
+  else
+    cg_update_span<false>(gp(P.op[4]) + base, gp(P.op[3]) + base, gp(P.op[2]) + base, gp(P.op[1]) + base,
+                          gp(P.ik) + base, N, mr, gmj, sq, a, rz, rr);
 }
 
 // One row j of the new direction, one wave: p = beta p + 1 z with z = r / sqrt(diag)
 // (pqdts.f90:2075) and, in reduced space, ev[j] = -(sum p[j,:] - p[j,maxr]) for
-// the next product.
+// the next product; with C32 the FP32 copy of p outside the implicit column and,
+// with LT (the late fold), the sum pb of p over the free entries, as p32_elem forms
+// them. Restrict-qualified arrays and batched loads, as in cg_update_span.
+template <bool C32, bool LT>
+__device__ __forceinline__ void cg_dir_span(GLOBAL double *__restrict__ p, const GLOBAL double *__restrict__ r,
+                                            const GLOBAL uint8_t *__restrict__ ik, GLOBAL float *__restrict__ p32,
+                                            int N, int mr, double sq, double beta, double &psum, double &own,
+                                            double &pb) {
+  for (int n0 = lane_id(); n0 < N; n0 += ROW_U * WAVE) {
+    double pv[ROW_U], rv[ROW_U];
+    uint8_t kv[ROW_U];
+#pragma unroll
+    for (int u = 0; u < ROW_U; u++) {
+      const int n = n0 + u * WAVE, nc = n < N ? n : N - 1;
+      pv[u] = p[nc];
+      rv[u] = r[nc];
+      kv[u] = (C32 && LT) ? ik[nc] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < ROW_U; u++) {
+      const int n = n0 + u * WAVE;
+      if (n >= N) break;
+      const double v = beta * pv[u] + 1.0 * (rv[u] / sq);
+      p[n] = v;
+      psum += v;
+      if (n == mr) own = v;
+      if (C32 && n != mr) {
+        p32[n] = (float)v;
+        if (LT && !kv[u]) pb += v;
+      }
+    }
+  }
+}
 __device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, bool late, double beta) {
   const int N = P.N;
   const int mr = full ? -1 : gp(P.maxr)[j];
@@ -1397,14 +1462,15 @@ __device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, bool
   const bool lt = late && !full;
   double psum = 0.0, own = 0.0, pb = 0.0;
   const int64_t base = (int64_t)j * N;
-  for (int n = lane_id(); n < N; n += WAVE) {
-    const int64_t idx = base + n;
-    const double p = beta * gp(P.op[3])[idx] + 1.0 * (gp(P.op[1])[idx] / sq);
-    gp(P.op[3])[idx] = p;
-    psum += p;
-    if (n == mr) own = p;
-    if (P.p32) p32_elem(P, idx, n, mr, lt, lt && gp(P.ik)[idx] != 0, p, pb);
-  }
+  GLOBAL double *pr = gp(P.op[3]) + base;
+  const GLOBAL double *rr = gp(P.op[1]) + base;
+  const GLOBAL uint8_t *ikr = gp(P.ik) + base;
+  if (!P.p32)
+    cg_dir_span<false, false>(pr, rr, ikr, nullptr, N, mr, sq, beta, psum, own, pb);
+  else if (lt)
+    cg_dir_span<true, true>(pr, rr, ikr, gp(P.p32) + base, N, mr, sq, beta, psum, own, pb);
+  else
+    cg_dir_span<true, false>(pr, rr, ikr, gp(P.p32) + base, N, mr, sq, beta, psum, own, pb);
   double evj = 0.0;
   if (!full) {
     psum = wave_sum(psum);
