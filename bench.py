@@ -207,12 +207,16 @@ def cpu_baseline_cfg2(prob, op_counts, threads):
 
 # ------------------------------------------------------------------ workloads
 HVP_SPLIT = -1  # --hvp-split: -1 leaves the library's choice
+POLISH = 0      # --polish: outer iterations of the polish after conv < eps (0: none)
+POLISH_CG = 4000  # --polish-cg: CG rounds of one polish iteration, at most
 
 
 def make_ctx(pqdts_amd, prob):
     c = pqdts_amd.Context.from_problem(prob)
     if HVP_SPLIT >= 0:
         c.set_hvp_split(HVP_SPLIT)
+    if POLISH > 0:
+        c.set_polish(POLISH, cg_max=POLISH_CG)
     return c
 
 
@@ -253,14 +257,17 @@ def run_batched(pqdts_amd, problems, ctxs, precision, maxiter, timing, tag, lane
             slot_of, steps = {}, 0
             for i in idx:
                 slot_of[b.admit(ctxs[i], 2, maxiter, problems[i].eps, 1, 0, precision, record=False)] = i
+            t_log = time.perf_counter()
             while slot_of:
                 for s in b.step():
                     i = slot_of.pop(s)
                     lat[i] = time.perf_counter() - t0
                     res[i] = b.result(s)
                     b.release(s)
+                    res[i]["polish"] = ctxs[i].polish_result()
                 steps += 1
-                if steps % 1000 == 0:
+                if steps % 1000 == 0 or time.perf_counter() - t_log > 30.0:
+                    t_log = time.perf_counter()
                     log(f"  {tag} lane {g}: {steps} steps, {len(slot_of)} unfinished, "
                         f"{time.perf_counter() - t0:.1f} s")
             log(f"  {tag} lane {g}: {len(idx)} problems done in {steps} steps, {time.perf_counter() - t0:.1f} s")
@@ -419,6 +426,24 @@ def determined_subset(args, pqdts_amd, problems, ctxs, eps=1e-11):
             "dpi_inf_max": max(dpi), "dpi_inf_median": pct(dpi, 50), "p99_latency_s": pct(lat_m, 99)}
 
 
+def polish_summary(res_m, res_b64, problems):
+    if POLISH == 0:
+        return None
+    out = {"max_outer": POLISH, "cg_max": POLISH_CG,
+           "not_converged": [{"M": problems[i].M, "N": problems[i].N, "gamma": problems[i].gamma,
+                              "mixed": res_m[i]["polish"], "fp64": res_b64[i]["polish"]}
+                             for i in range(len(problems))
+                             if res_m[i]["polish"]["status"] != 1 or res_b64[i]["polish"]["status"] != 1][:12]}
+    for tag, res in (("mixed", res_m), ("fp64", res_b64)):
+        pr = [r["polish"] for r in res]
+        out[tag] = {"certified": sum(1 for q in pr if q["status"] == 1), "of": len(pr),
+                    "iterations": int(sum(q["iterations"] for q in pr)),
+                    "max_iterations": int(max(q["iterations"] for q in pr)),
+                    "cg_rounds": int(sum(q["cg_rounds"] for q in pr)),
+                    "max_cg_rounds": int(max(q["cg_rounds"] for q in pr))}
+    return out
+
+
 def workload_mix(args, pqdts_amd, workloads):
     T_START = time.time()
     problems = workloads.batch_mix(args.count, eps=args.eps)
@@ -488,6 +513,8 @@ def workload_mix(args, pqdts_amd, workloads):
     for c, x in zip(ctxs, X_b64):
         c.put("X", x)
         L64.append(objective(c))
+    # converged: the reference's test conv < eps (after the polish, if on, which also
+    # reports whether it certified the minimiser)
     conv_m = [r["conv"] < p.eps for r, p in zip(res_m, problems)]
     conv_64 = [r["conv"] < p.eps for r, p in zip(res_b64, problems)]
     dpi = [float(np.max(np.abs(a - b))) for a, b in zip(X_m, X_b64)]
@@ -510,7 +537,7 @@ def workload_mix(args, pqdts_amd, workloads):
             mid_counts = {k: after[k][0] - base.get(k, (0, 0))[0] for k in after}
         conv_p.append(conv < p.eps)
         same_bits += int(np.array_equal(c.get("X"), X_b64[i]))
-        if n % 8 == 0:
+        if n % 8 == 0 or POLISH > 0:
             log(f"  plain fp64 {n + 1}/{len(seq_idx)}: M={p.M} N={p.N} {seq_t[i]:.3f} s")
     seq_total = sum(seq_t.values())
     plain_rate = sum(conv_p) / seq_total
@@ -525,7 +552,7 @@ def workload_mix(args, pqdts_amd, workloads):
     for c in sctx:
         c.set_hvp_path("auto")
     floor = [float(np.max(np.abs(X_b64[i] - x))) for i, x in zip(seq_idx, X_alt)]
-    det = determined_subset(args, pqdts_amd, problems, ctxs)
+    det = determined_subset(args, pqdts_amd, problems, ctxs) if POLISH == 0 else None
     nconv_m = sum(conv_m)
     value = steps * sum(gated) / total
     seq_lat = np.cumsum([seq_t[i] for i in seq_idx]) * args.seq_stride
@@ -574,8 +601,10 @@ def workload_mix(args, pqdts_amd, workloads):
                     "sum in a different order (Gram vs F products): how far eps leaves Pi undetermined; the 1e-6 "
                     "gate cannot hold where FP64 itself does not"},
         "determined_subset": det,
+        "polish": polish_summary(res_m, res_b64, problems),
         "slowest_mixed": [{"M": problems[i].M, "N": problems[i].N, "gamma": problems[i].gamma,
-                           "newton": res_m[i]["newton"], "pcg": res_m[i]["pcg"], "latency_s": lat_m[i]}
+                           "newton": res_m[i]["newton"], "pcg": res_m[i]["pcg"], "latency_s": lat_m[i],
+                           "polish": res_m[i].get("polish")}
                           for i in sorted(range(len(problems)), key=lambda k: -lat_m[k])[:5]],
         "solve_stats_mixed": {"newton": int(sum(r["newton"] for r in res_m)),
                               "pcg": int(sum(r["pcg"] for r in res_m)),
@@ -860,11 +889,16 @@ def main():
                     help="wall-clock seconds for the timed steps (mix): steps run while they fit")
     ap.add_argument("--kernels", type=int, default=0)
     ap.add_argument("--calib-copy", action="store_true")
+    ap.add_argument("--polish", type=int, default=0,
+                    help="polish every solve to the problem's minimiser after conv < eps, at most this many "
+                         "primal-dual active-set iterations (0: off)")
+    ap.add_argument("--polish-cg", type=int, default=4000, help="CG rounds of one polish iteration, at most")
     ap.add_argument("--hvp-split", type=int, default=-1,
                     help="split of the Gram product over band chunks (0 by shape, 1 none, k fixed)")
     args = ap.parse_args()
-    global HVP_SPLIT
+    global HVP_SPLIT, POLISH, POLISH_CG
     HVP_SPLIT = args.hvp_split
+    POLISH, POLISH_CG = args.polish, args.polish_cg
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems would "
               f"shard one per GPU with no communication, so N GPUs serve N times the problems of one",

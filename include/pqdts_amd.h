@@ -58,7 +58,10 @@ enum pqdts_event {
   PQDTS_EV_STOP1 = 4,   /* dlalpha                                             (pqdts.f90:1956) */
   PQDTS_EV_ITER0 = 5,   /* L0 sqrt(||der||^2)                                  (pqdts.f90:1914) */
   PQDTS_EV_LAMBDA = 6,  /* i lambda min(G+lambda)                              (pqdts.f90:1059) */
-  PQDTS_EV_STAGE = 7    /* oiter conv: end of a stage, before its output file  (pqdts.f90:2139) */
+  PQDTS_EV_STAGE = 7,   /* oiter conv: end of a stage, before its output file  (pqdts.f90:2139) */
+  /* polish (pqdts_set_polish; no counterpart in the reference) */
+  PQDTS_EV_POLISH = 8,  /* iteration active_entries changed_entries cg_rounds step_max          */
+  PQDTS_EV_POLISHED = 9 /* status(1 converged, 2 cap) iterations cg_rounds conv L               */
 };
 typedef void (*pqdts_event_cb)(void *user, int kind, const double *values, int nvalues);
 
@@ -140,6 +143,29 @@ int pqdts_lalpha_min_auglag(pqdts_ctx *ctx, double *alpha, double *dlalpha);
  * same context to write the per-stage file. *conv receives the last KKT value. */
 int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_stage, int output,
                 pqdts_event_cb cb, void *user, double *conv);
+
+/* Polish (no counterpart in the reference). The reference stops at conv < eps
+ * (pqdts.f90:1034-1067, 2134); conv weights each KKT residual by X, so on
+ * problems with weak curvature (small gamma) points far apart pass the same eps.
+ * With max_outer > 0, every later pqdts_solve and batch admission of this
+ * context continues from the minimisation's result with a primal-dual
+ * active-set Newton iteration on the reduced problem (the implicit column of
+ * each row is its largest entry): an entry is held at 0 when G - d X > 0 (G the
+ * reduced gradient, d the reduced Hessian diagonal of its row); the free entries
+ * take a full Newton step, solved by CG with the Jacobi preconditioner (at most
+ * cg_max rounds per iteration) to the relative residual tol / (the last step's
+ * largest entry), kept within [cg_rtol, 1e-3] (cg_rtol for the first step). It
+ * stops when the active set repeats and the last solve's error estimate (its
+ * relative residual times its step) is <= tol: a KKT point of the strictly
+ * convex problem, i.e. its unique minimiser to about tol; or after max_outer
+ * iterations. Gradient, active set, steps and the CG scalars are FP64; the
+ * Hessian products are FP64 unless fp32 = 1 and the solve is mixed precision.
+ * max_outer = 0 turns it off (the default). */
+int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, int cg_max, int fp32);
+/* outcome of the last polish on ctx: stats[0..2] = status (0 not run, 1 converged,
+ * 2 stopped at max_outer), iterations, CG rounds; *nact active entries, *step
+ * the last step's largest entry (each pointer may be NULL) */
+int pqdts_polish_result(pqdts_ctx *ctx, int *stats, int64_t *nact, double *step);
 
 /* The named sections timed on this context so far, under the reference's timer
  * names (pqdts.f90:103-186): min, L, L_O, dL, dL_O, dL_dLdX, BF, BF_O, BF_BX, CG,

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libpqdts_amd.so")
 
 SLOTS = {"X": 0, "G": 1, "HP": 2, "XTMP": 3, "SN": 4, "Z": 5}
 ROW_MAJOR, COL_MAJOR = 0, 1
-EV_ITER, EV_PCG, EV_LS, EV_ACTIVE, EV_STOP1, EV_ITER0, EV_LAMBDA, EV_STAGE = range(8)
+EV_ITER, EV_PCG, EV_LS, EV_ACTIVE, EV_STOP1, EV_ITER0, EV_LAMBDA, EV_STAGE, EV_POLISH, EV_POLISHED = range(10)
 EVENT_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int)
 
 # every symbol the header declares, for the load check
@@ -32,6 +32,7 @@ EXPORTS = [
     "pqdts_device_cu_count", "pqdts_batch_set_cu_mask", "simplexproj_condat_",
     "pqdts_simplexproj_rows_noalloc", "pqdts_server_create", "pqdts_server_destroy", "pqdts_server_submit",
     "pqdts_server_wait", "pqdts_server_result", "pqdts_server_info", "pqdts_footprint", "pqdts_device_timers",
+    "pqdts_set_polish", "pqdts_polish_result",
 ]
 
 _lib = None
@@ -103,6 +104,8 @@ def load():
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
     L.pqdts_set_hvp_path.argtypes = [vp, i32]
     L.pqdts_set_hvp_split.argtypes = [vp, i32]
+    L.pqdts_set_polish.argtypes = [vp, i32, d, d, i32, i32]
+    L.pqdts_polish_result.argtypes = [vp, C.POINTER(i32), i64p, dp]
     L.pqdts_batch_create.argtypes = [i32, C.POINTER(vp)]
     L.pqdts_batch_destroy.argtypes = [vp]
     L.pqdts_batch_admit.argtypes = [vp, vp, i32, i32, d, i32, i32, i32, EVENT_CB, vp, C.POINTER(i32)]
@@ -339,6 +342,19 @@ class Context:
     def set_hvp_split(self, split: int):
         """Split of the Gram product over band chunks: 0 by shape, 1 none, 2..16 fixed."""
         _check(load().pqdts_set_hvp_split(self.h, int(split)))
+
+    def set_polish(self, max_outer=100, cg_rtol=1e-6, tol=1e-10, cg_max=20000, fp32=False):
+        """Polish every later solve of this context to the problem's minimiser (primal-dual
+        active-set Newton after conv < eps) to about `tol`; max_outer = 0 turns it off."""
+        _check(load().pqdts_set_polish(self.h, int(max_outer), float(cg_rtol), float(tol), int(cg_max),
+                                       int(bool(fp32))))
+
+    def polish_result(self) -> dict:
+        st = (C.c_int * 3)()
+        nact = C.c_int64()
+        step = C.c_double()
+        _check(load().pqdts_polish_result(self.h, st, C.byref(nact), C.byref(step)))
+        return {"status": st[0], "iterations": st[1], "cg_rounds": st[2], "nact": nact.value, "step": step.value}
 
     def kernel_timing(self, on):
         """True / 1: time every launch; k > 1: every k-th launch; False / 0: off."""

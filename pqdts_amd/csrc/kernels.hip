@@ -83,13 +83,18 @@ constexpr int CG_LOG_ROUNDS = 50;
 
 constexpr int NREG = 4;
 constexpr int LS_K = 2;                // line-search trials evaluated per host round trip, at most
-constexpr int NSCAL = 16 + 3 * LS_K;   // solver scalars, then per-trial objective terms and xmin
+constexpr int NSCAL = 17 + 3 * LS_K;   // solver scalars, then per-trial objective terms and xmin
 
 // flags of a problem within one launch
 constexpr unsigned F_FULL = 1u;  // full space (tfullspace); otherwise reduced
 // Mixed-precision CG of a reduced problem: the Hessian product stores its result
 // unfolded with the fold value of each row (gm), and kb_cg_update applies the fold
 constexpr unsigned F_LATE = 2u;
+// Polish iteration of a reduced problem (primal-dual active-set Newton after the
+// minimisation): the CG keeps the active set it is given (ik), uses the Jacobi
+// preconditioner z = r / diag and stops at a relative residual (the launch's
+// per-problem argument of kb_cg_start)
+constexpr unsigned F_POLISH = 4u;
 
 struct WorkList {
   int np;
@@ -139,7 +144,9 @@ __device__ __forceinline__ int wl_find(const WorkList &wl, int bid) {
   const bool full = (wl.flags[wi_] & F_FULL) != 0;         \
   const double arg = wl.a[wi_];                            \
   const bool late = (wl.flags[wi_] & F_LATE) != 0;         \
+  const bool polish = (wl.flags[wi_] & F_POLISH) != 0;     \
   (void)late;                                              \
+  (void)polish;                                            \
   (void)full;                                              \
   (void)nblk;                                              \
   (void)arg
@@ -1309,6 +1316,13 @@ __device__ __forceinline__ void p32_row_end(const Prob &P, int j, int mr, bool l
   }
 }
 
+// Preconditioner divisor of row j: sqrt(diag) in the reference's PCG
+// (pqdts.f90:2028, 2067), diag itself (Jacobi) in a polish iteration.
+__device__ __forceinline__ double prec_div(const Prob &P, int j, bool full, bool polish) {
+  const double d = full ? gp(P.diag_full)[j] : gp(P.diag_red)[j];
+  return polish ? d : sqrt(d);
+}
+
 // Active set + CG start (pqdts.f90:1986-2031): ik = X < tol and G > 0, G[ik] = 0;
 // partials of nact (region 0), |G|^2 (region 1) and r.z (region 2); r = -G;
 // p = z = r / sqrt(diag) (z itself is not stored); sn = 0;
@@ -1318,7 +1332,7 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
   ROW_PROLOGUE();
   double nact = 0.0, nd = 0.0, b1 = 0.0;
   if (j < P.M) {
-    const double sq = sqrt(full ? gp(P.diag_full)[j] : gp(P.diag_red)[j]);
+    const double sq = prec_div(P, j, full, polish);
     const int mr = full ? -1 : gp(P.maxr)[j];
     double psum = 0.0, own = 0.0, pb = 0.0;
     const bool lt = late && !full;
@@ -1327,7 +1341,14 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_init(const Prob *__restrict__ pro
       const int64_t idx = base + n;
       double g = gp(P.op[1])[idx];
       uint8_t a = 0;
-      if (tactive && gp(P.op[0])[idx] < kActiveTol && g > 0.0) {
+      if (polish) {
+        // the active set of the polish iteration, chosen by kb_pdas_active
+        a = gp(P.ik)[idx];
+        if (a) {
+          g = 0.0;
+          nact += 1.0;
+        }
+      } else if (tactive && gp(P.op[0])[idx] < kActiveTol && g > 0.0) {
         a = 1;
         g = 0.0;
         nact += 1.0;
@@ -1405,10 +1426,10 @@ __device__ __forceinline__ void cg_update_span(GLOBAL double *__restrict__ sn, c
     }
   }
 }
-__device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, bool lf, double a, double &rz,
-                                              double &rr) {
+__device__ __forceinline__ void cg_update_row(const Prob &P, int j, bool full, bool lf, bool pol, double a,
+                                              double &rz, double &rr) {
   const int N = P.N;
-  const double sq = sqrt(full ? gp(P.diag_full)[j] : gp(P.diag_red)[j]);
+  const double sq = prec_div(P, j, full, pol);
   const int mr = lf ? gp(P.maxr)[j] : -1;
   const double gmj = lf ? gp(P.gm)[j] : 0.0;
   const int64_t base = (int64_t)j * N;
@@ -1455,10 +1476,10 @@ __device__ __forceinline__ void cg_dir_span(GLOBAL double *__restrict__ p, const
     }
   }
 }
-__device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, bool late, double beta) {
+__device__ __forceinline__ void cg_dir_row(const Prob &P, int j, bool full, bool late, bool pol, double beta) {
   const int N = P.N;
   const int mr = full ? -1 : gp(P.maxr)[j];
-  const double sq = sqrt(full ? gp(P.diag_full)[j] : gp(P.diag_red)[j]);
+  const double sq = prec_div(P, j, full, pol);
   const bool lt = late && !full;
   double psum = 0.0, own = 0.0, pb = 0.0;
   const int64_t base = (int64_t)j * N;
@@ -1494,7 +1515,7 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_update(const Prob *__restrict__ p
   if (threadIdx.x == 0 && lb == 0) gp(P.scal)[i_v2] = v2;
   const double a = gp(P.scal)[i_b1] / v2;
   double rz = 0.0, rr = 0.0;
-  if (j < P.M) cg_update_row(P, j, full, late && !full, a, rz, rr);
+  if (j < P.M) cg_update_row(P, j, full, late && !full, polish, a, rz, rr);
   double t = block_sum(rz, sh);
   if (threadIdx.x == 0) gp(P.part)[lb] = t;
   t = block_sum(rr, sh);
@@ -1521,29 +1542,34 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_dir(const Prob *__restrict__ prob
   const double b2 = sqrt(vrr);
   const bool stop = b2 <= sc[i_eps];
   if (threadIdx.x == 0 && lb == 0) {
-    GLOBAL double *row = gp(P.cglog) + (int64_t)(j_round - 1) * 5;
-    row[0] = (double)j_round;
-    row[1] = b2;
-    row[2] = sc[i_eps];
-    row[3] = sc[i_b1];
-    row[4] = vrz;
+    if (j_round >= 1 && j_round <= CG_LOG_ROUNDS) {
+      GLOBAL double *row = gp(P.cglog) + (int64_t)(j_round - 1) * 5;
+      row[0] = (double)j_round;
+      row[1] = b2;
+      row[2] = sc[i_eps];
+      row[3] = sc[i_b1];
+      row[4] = vrz;
+    }
     sc[i_rz] = vrz;
     sc[i_rr] = vrr;
     sc[i_n] = (double)j_round;
     if (stop) sc[i_act] = 0.0;
   }
   if (stop || j >= P.M) return;
-  cg_dir_row(P, j, full, late, vrz / sc[i_b1]);
+  cg_dir_row(P, j, full, late, polish, vrz / sc[i_b1]);
 }
 
 // Start of PCG (pqdts.f90:2024-2031), one thread block per problem: sums of
 // kb_cg_init's partials (nact, |G|^2, r.z) into scal[i_nact], scal[i_nd],
-// scal[i_b1]; CG tolerance eps = min(0.5, sqrt(|G|)) |G|; CG marked running
+// scal[i_b1]; CG tolerance eps = min(0.5, sqrt(|G|)) |G| (a polish iteration:
+// its relative residual, the launch argument, times |G|); CG marked running
 // with no rounds done.
 __global__ __launch_bounds__(BLOCK) void kb_cg_start(const Prob *__restrict__ probs, WorkList wl, int i_nact,
                                                      int i_nd, int i_b1, int i_eps, int i_act, int i_n) {
   __shared__ double sh[WAVES_PER_BLOCK];
   const Prob &P = probs[wl.pid[blockIdx.x]];
+  const bool polish = (wl.flags[blockIdx.x] & F_POLISH) != 0;
+  const double rtol = wl.a[blockIdx.x];
   const int n = gp(P.pn)[0];
   const double nact = reduce_parts(gp(P.part), n, false, sh);
   const double nd = reduce_parts(gp(P.part) + (int64_t)P.pcap, n, false, sh);
@@ -1553,10 +1579,85 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_start(const Prob *__restrict__ pr
     sc[i_nact] = nact;
     sc[i_nd] = nd;
     sc[i_b1] = b1;
-    sc[i_eps] = fmin(0.5, sqrt(sqrt(nd))) * sqrt(nd);
+    sc[i_eps] = polish ? rtol * sqrt(nd) : fmin(0.5, sqrt(sqrt(nd))) * sqrt(nd);
     sc[i_act] = 1.0;
     sc[i_n] = 0.0;
   }
+}
+
+// A CG that continues into the next batch step (a polish iteration's CG runs at
+// most one step's rounds at a time): the round count restarts at 1, so the r.z of
+// the last round done moves to the slot round 1 reads (i_b1). Thread 0 per problem.
+__global__ void kb_cg_carry(const Prob *__restrict__ probs, WorkList wl, int i_n, int i_rz0, int i_rz1, int i_b1) {
+  if (threadIdx.x != 0) return;
+  GLOBAL double *sc = gp(probs[wl.pid[blockIdx.x]].scal);
+  const int n = (int)sc[i_n];
+  if (n >= 1) sc[i_b1] = sc[(n & 1) ? i_rz1 : i_rz0];
+  sc[i_n] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Polish: primal-dual active-set Newton on the reduced problem (no counterpart in
+// the reference). With G the reduced gradient (the multiplier estimate of each
+// entry: G = g - g[maxr], zero on the free entries at the optimum) and d the
+// reduced Hessian diagonal of the row, an entry outside the implicit column is
+// active (held at 0) when G - d X > 0. Such a fixed point with X >= 0 on the free
+// entries satisfies the KKT conditions of the (strictly convex) problem.
+// ---------------------------------------------------------------------------
+
+// One wave per row: the new active set into ik, X = 0 on it; partials of the
+// entries whose activity changed (region 0) and of the active entries (region 1).
+__global__ __launch_bounds__(BLOCK) void kb_pdas_active(const Prob *__restrict__ probs, WorkList wl) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  ROW_PROLOGUE();
+  double nchg = 0.0, nact = 0.0;
+  if (j < P.M) {
+    const double d = gp(P.diag_red)[j];
+    const int mr = gp(P.maxr)[j];
+    const int64_t base = (int64_t)j * N;
+    GLOBAL double *xr = gp(P.op[0]) + base;
+    const GLOBAL double *gr = gp(P.op[1]) + base;
+    GLOBAL uint8_t *ikr = gp(P.ik) + base;
+    for (int n = lane_id(); n < N; n += WAVE) {
+      const double x = xr[n];
+      const uint8_t a = (n != mr && gr[n] - d * x > 0.0) ? 1 : 0;
+      if (a != ikr[n]) nchg += 1.0;
+      ikr[n] = a;
+      if (a) {
+        nact += 1.0;
+        xr[n] = 0.0;
+      }
+    }
+  }
+  double t = block_sum(nchg, sh);
+  if (threadIdx.x == 0) gp(P.part)[lb] = t;
+  t = block_sum(nact, sh);
+  if (threadIdx.x == 0) gp(P.part)[(int64_t)P.pcap + lb] = t;
+  if (threadIdx.x == 0 && lb == 0) gp(P.pn)[0] = gp(P.pn)[1] = nblk;
+}
+
+// One wave per row: the full Newton step X += SN outside the implicit column (SN
+// is 0 on the active set); per block the partial -max |SN| (region), so that the
+// minimum over the blocks is minus the step's largest entry.
+__global__ __launch_bounds__(BLOCK) void kb_pdas_step(const Prob *__restrict__ probs, WorkList wl, int region) {
+  __shared__ double sh[WAVES_PER_BLOCK];
+  ROW_PROLOGUE();
+  double m = 0.0;
+  if (j < P.M) {
+    const int mr = gp(P.maxr)[j];
+    const int64_t base = (int64_t)j * N;
+    GLOBAL double *xr = gp(P.op[0]) + base;
+    const GLOBAL double *sr = gp(P.op[4]) + base;
+    for (int n = lane_id(); n < N; n += WAVE) {
+      if (n == mr) continue;
+      const double v = sr[n];
+      xr[n] = xr[n] + v;
+      m = fmax(m, fabs(v));
+    }
+  }
+  const double b = block_min(-m, sh);
+  if (threadIdx.x == 0) gp(P.part)[(int64_t)region * P.pcap + lb] = b;
+  if (threadIdx.x == 0 && lb == 0) gp(P.pn)[region] = nblk;
 }
 
 // Two-metric diagonal step (pqdts.f90:2095-2102): sn = -G/diag on active entries

@@ -61,17 +61,20 @@ int guarded(F &&f) {
 }
 
 constexpr int TILE = 64;  // rows and columns of a band-GEMM block tile
+// loosest relative CG residual of a polish iteration
+constexpr double kPolishLooseRtol = 1e-3;
 constexpr int KC = 16;    // band chunk staged per LDS buffer (4 MFMA k-steps)
 
 // device scalar slots per problem
 enum {
   S_LA = 0, S_LB, S_XMIN, S_NACT, S_ND, S_B1, S_V2, S_RZ0, S_RZ1, S_RR, S_CONV, S_DOT,
   S_EPSCG, S_CGACT, S_CGN,  // device-side CG loop: eps, still-iterating flag, rounds done
-  S_LT = 16,                 // line-search trial k: data term S_LT + k, smoothing S_LTB + k, xmin S_LTX + k
+  S_NCHG, S_STEP,            // polish: active entries that changed, minus the step's largest entry
+  S_LT = 17,                 // line-search trial k: data term S_LT + k, smoothing S_LTB + k, xmin S_LTX + k
   S_LTB = S_LT + LS_K,
   S_LTX = S_LTB + LS_K
 };
-static_assert(S_CGN < S_LT && S_LTX + LS_K <= NSCAL, "scalar slots");
+static_assert(S_STEP < S_LT && S_LTX + LS_K <= NSCAL, "scalar slots");
 // partial-sum regions per problem
 enum { R_A = 0, R_B = 1, R_C = 2 };
 
@@ -158,6 +161,20 @@ struct pqdts_ctx {
   float *p32 = nullptr;      // FP32 expanded CG direction (mixed precision; Prob::p32)
   double *pb = nullptr;      // Prob::pb (mixed precision)
   bool use_gram() const { return hvp_pref == 2 || (hvp_pref == 0 && gram_auto); }
+  // polish after the minimisation (pqdts_set_polish) and the outcome of the last one
+  struct Polish {
+    int max_outer = 0;        // 0: no polish
+    double cg_rtol = 1e-6;    // the tightest relative CG residual
+    double tol = 1e-10;       // target distance of the result from the minimiser
+    int cg_max = 20000;       // CG rounds of one outer iteration, at most
+    int fp32 = 0;             // 1: FP32 Hessian products where the solve is mixed
+  } pol;
+  struct PolishOut {
+    int status = 0;           // 0 not run, 1 converged, 2 stopped at the iteration cap
+    int iters = 0, rounds = 0;
+    int64_t nact = 0;
+    double step = 0.0;
+  } pol_out;
   std::unique_ptr<Engine> eng;  // standalone engine (this problem alone, on stream s)
   Engine *batch = nullptr;      // the batch engine whose slot holds this context, if any
   Engine &engine();
@@ -319,8 +336,8 @@ struct Engine {
       const pqdts_ctx *c = ctx[p];
       key.push_back(p);
       key.push_back(gen[p]);
-      key.push_back((c->full ? 1 : 0) | (late(p) ? 2 : 0) | (fp32[p] ? 4 : 0) | (c->use_gram() ? 8 : 0) |
-                    ((int64_t)c->hs << 4) | ((int64_t)c->hs_pref << 12));
+      key.push_back((c->full ? 1 : 0) | (late(p) ? 2 : 0) | (prod32(p) ? 4 : 0) | (c->use_gram() ? 8 : 0) |
+                    ((int64_t)c->hs << 4) | ((int64_t)c->hs_pref << 12) | ((int64_t)polishing(p) << 20));
     }
     auto it = cg_graphs.find(key);
     if (it == cg_graphs.end()) {
@@ -552,7 +569,7 @@ struct Engine {
         wl.pid[i] = p;
         wl.boff[i] = tot;
         tot += nb(p);
-        wl.flags[i] = (ctx[p]->full ? F_FULL : 0) | (late(p) ? F_LATE : 0);
+        wl.flags[i] = (ctx[p]->full ? F_FULL : 0) | (late(p) ? F_LATE : 0) | (polishing(p) ? F_POLISH : 0);
         wl.a[i] = arg[p];
       }
       wl.boff[wl.np] = tot;
@@ -577,12 +594,17 @@ struct Engine {
     return cdiv(c->N, TILE) * nt;
   }
 
+  // FP32 Hessian products for problem p: a mixed-precision slot, except while it
+  // polishes with FP64 products; polishing(p): in its polish phase
+  bool prod32(int p) const;
+  bool polishing(int p) const;
+
   // Inside the CG of a mixed-precision solve (cg_late), the fold of the Hessian
   // product moves from the product's epilogue to kb_cg_update (F_LATE), except
   // for an explicitly split Gram product, whose summing pass folds.
   bool cg_late = false;
   bool late(int p) const {
-    return cg_late && fp32[p] && !(ctx[p]->use_gram() && ctx[p]->hs > 1 && ctx[p]->hs_pref > 0);
+    return cg_late && prod32(p) && !(ctx[p]->use_gram() && ctx[p]->hs > 1 && ctx[p]->hs_pref > 0);
   }
   std::vector<int> early_of(const std::vector<int> &S) const {
     std::vector<int> r;
@@ -679,7 +701,7 @@ struct Engine {
     std::vector<int> g64, g32, f64, f32;
     for (int p : S) {
       const bool gram = ctx[p]->use_gram();
-      (gram ? (fp32[p] ? g32 : g64) : (fp32[p] ? f32 : f64)).push_back(p);
+      (gram ? (prod32(p) ? g32 : g64) : (prod32(p) ? f32 : f64)).push_back(p);
     }
     // Gram path: fold values, then one product 2 G p~ with the epilogue.
     // Problems with a split product write partials instead; their summing pass
@@ -688,7 +710,7 @@ struct Engine {
     std::vector<int> g64s, g32s;
     auto split_off = [&](std::vector<int> &v, std::vector<int> &sv) {
       std::vector<int> keep;
-      for (int p : v) (ctx[p]->hs > 1 && (!fp32[p] || ctx[p]->hs_pref > 0) ? sv : keep).push_back(p);
+      for (int p : v) (ctx[p]->hs > 1 && (!prod32(p) || ctx[p]->hs_pref > 0) ? sv : keep).push_back(p);
       v.swap(keep);
     };
     split_off(g64, g64s);
@@ -904,6 +926,18 @@ struct PState {
   double ls_L0 = 0.0, ls_df = 0.0, ls_alpha = 0.0, ls_lm = 0.0, ls_amin = 0.0;
   // counters
   int newton = 0, pcg = 0, trials = 0;
+  // polish (phase 3): settings copied from the context, then the iteration state
+  pqdts_ctx::Polish pol;
+  int pol_it = 0;         // outer iterations started
+  int pol_cgr = 0;        // CG rounds of the current outer iteration
+  int pol_rounds = 0;     // CG rounds of all outer iterations
+  bool pol_incg = false;  // the current outer iteration's CG continues next step
+  bool pol_entered = false;
+  double pol_rtol = 1.0;  // relative CG residual of the last solve
+  double pol_step = 0.0;  // largest entry of the last step
+  int pol_nchg = 0;
+  int64_t pol_nact = 0;
+  int pol_status = 0;
   void emit(int kind, std::initializer_list<double> v) {
     if (cb) {
       std::vector<double> t(v);
@@ -927,6 +961,9 @@ Engine::~Engine() {
   if (hlog) (void)hipHostFree(hlog);
   if (own_stream && s) (void)hipStreamDestroy(s);
 }
+
+bool Engine::polishing(int p) const { return st[p] && st[p]->phase == 3; }
+bool Engine::prod32(int p) const { return fp32[p] && !(polishing(p) && !st[p]->pol.fp32); }
 
 bool Engine::any_active() const {
   for (int p = 0; p < cap; p++)
@@ -1062,7 +1099,7 @@ void Engine::tick() {
       q.emit(PQDTS_EV_STAGE, {(double)q.stage, q.conv});
       q.stage++;
       if (q.stage > 2) {
-        q.phase = 2;
+        q.phase = q.pol.max_outer > 0 ? 3 : 2;
       } else {
         q.iiter = 0;
         q.dlalpha = -std::numeric_limits<double>::max();
@@ -1071,29 +1108,98 @@ void Engine::tick() {
     }
   }
   std::vector<int> A, A1, A2;
-  for (int p = 0; p < cap; p++)
-    if (ctx[p] && st[p] && st[p]->phase == 1) {
-      A.push_back(p);
-      PState &q = *st[p];
-      q.iiter++;
-      q.newton++;
-      if (q.mode == 1) {
-        ctx[p]->full = false;
-        A1.push_back(p);
-      } else {
-        ctx[p]->full = (q.stage == 1);
-        A2.push_back(p);
-      }
+  std::vector<int> Q;  // polishing
+  for (int p = 0; p < cap; p++) {
+    if (!ctx[p] || !st[p]) continue;
+    if (st[p]->phase == 3) {
+      Q.push_back(p);
+      continue;
     }
-  if (A.empty()) return;
+    if (st[p]->phase != 1) continue;
+    A.push_back(p);
+    PState &q = *st[p];
+    q.iiter++;
+    q.newton++;
+    if (q.mode == 1) {
+      ctx[p]->full = false;
+      A1.push_back(p);
+    } else {
+      ctx[p]->full = (q.stage == 1);
+      A2.push_back(p);
+    }
+  }
+  if (A.empty() && Q.empty()) return;
   if (!A2.empty()) transform(A2, true);   // pqdts.f90:1970-1983
   if (!A1.empty()) transform(A1, false);  // pqdts.f90:1940-1944
-  dL(A, X, G);
-  if (!A2.empty()) {
+  // Polish (no counterpart in the reference): problems starting an outer iteration
+  // (Qs; on entry the reduced-space transform of the minimisation's result),
+  // problems whose CG continues from the last step (Qc)
+  std::vector<int> Qs, Qc, Qent;
+  for (int p : Q) {
+    PState &q = *st[p];
+    (q.pol_incg ? Qc : Qs).push_back(p);
+    if (!q.pol_entered) {
+      q.pol_entered = true;
+      ctx[p]->full = false;
+      Qent.push_back(p);
+    }
+  }
+  if (!Qent.empty()) transform(Qent, true);
+  std::vector<int> AQ = A;
+  AQ.insert(AQ.end(), Qs.begin(), Qs.end());
+  if (!AQ.empty()) dL(AQ, X, G);
+  std::vector<int> Qn, Fz;  // polish: new outer iteration, finished
+  if (!Qs.empty()) {
+    Scope sc(this, "polish");
+    launch(Qs, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kb_pdas_active<<<tot, BLOCK, 0, s>>>(dprobs, wl);
+      kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 2, R_A, S_NCHG, R_B, S_NACT, 0, 0, 0);
+    });
+    fetch();
+    for (int p : Qs) {
+      PState &q = *st[p];
+      const double *h = host_scal(p);
+      q.pol_nchg = (int)h[S_NCHG];
+      q.pol_nact = (int64_t)h[S_NACT];
+      // converged: the active set repeats (the last solve was on this same face)
+      // and that solve's error, about its relative residual times its step, is
+      // within the target
+      if (q.pol_it > 0 && q.pol_nchg == 0 && q.pol_rtol * q.pol_step <= q.pol.tol) {
+        q.pol_status = 1;
+        Fz.push_back(p);
+      } else if (q.pol_it >= q.pol.max_outer) {
+        q.pol_status = 2;
+        Fz.push_back(p);
+      } else {
+        // each step is solved to the relative residual that makes its error about
+        // the target, given the last step's size: tight while the steps are large
+        // (an inexact step there can make the active set diverge), looser once
+        // only a small correction is left
+        q.pol_it++;
+        q.pol_cgr = 0;
+        q.pol_incg = true;
+        const double want = q.pol_it == 1 ? q.pol.cg_rtol : q.pol.tol / std::max(q.pol_step, 1e-300);
+        q.pol_rtol = std::min(kPolishLooseRtol, std::max(q.pol.cg_rtol, want));
+        arg[p] = q.pol_rtol;
+        Qn.push_back(p);
+      }
+    }
+    // X changed on the new active set
+    if (!Qn.empty()) dL(Qn, X, G);
+  }
+  std::vector<int> CG0 = A2;  // CG from a fresh start
+  CG0.insert(CG0.end(), Qn.begin(), Qn.end());
+  std::vector<int> Qstep;     // polish problems whose CG ended in this step
+  if (!CG0.empty() || !Qc.empty()) {
     cg_late = true;
-    cg_init(A2);  // pqdts.f90:1986-2031
+    if (!CG0.empty()) cg_init(CG0);  // pqdts.f90:1986-2031
+    if (!Qc.empty())
+      launch(Qc, [](int) { return 1; }, [&](const WorkList &wl, int) {
+        kb_cg_carry<<<wl.np, 64, 0, s>>>(dprobs, wl, S_CGN, S_RZ0, S_RZ1, S_B1);
+      });
     // PCG, pqdts.f90:2036-2090. The exit test runs on the device: every CG launch
     // skips problems whose flag dropped, and the host only checks every few rounds.
+    // A polish CG runs at most 50 rounds per step too and continues in the next.
     {
       Scope sc(this, "CG");
       guard = S_CGACT;
@@ -1101,7 +1207,8 @@ void Engine::tick() {
       auto ib1_of = [](int j) { return j == 1 ? (int)S_B1 : ((j - 1) & 1) ? (int)S_RZ1 : (int)S_RZ0; };
       // problems still in CG; those whose CG stopped leave the launches at the
       // next host check (their blocks would only exit at the guard)
-      std::vector<int> C = A2;
+      std::vector<int> C = CG0;
+      C.insert(C.end(), Qc.begin(), Qc.end());
       auto rounds = [&](int j0, int j1) {
         for (int j = j0; j <= j1; j++) {
           kround = j;
@@ -1131,8 +1238,10 @@ void Engine::tick() {
     cg_late = false;
     fetch();
     kresolve();
-    HIPC(hipMemcpyAsync(hlog, dlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipMemcpyDeviceToHost, s));
-    HIPC(hipStreamSynchronize(s));
+    if (!A2.empty()) {
+      HIPC(hipMemcpyAsync(hlog, dlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipMemcpyDeviceToHost, s));
+      HIPC(hipStreamSynchronize(s));
+    }
     for (int p : A2) {
       PState &q = *st[p];
       const double *h = host_scal(p);
@@ -1143,12 +1252,35 @@ void Engine::tick() {
                                                          lg[5 * r + 4]});
       q.pcg += nr;
     }
-    dL(A2, X, G);       // pqdts.f90:2093
-    twometric(A2);      // pqdts.f90:2095-2102
+    std::vector<int> QC = Qn;
+    QC.insert(QC.end(), Qc.begin(), Qc.end());
+    for (int p : QC) {
+      PState &q = *st[p];
+      const double *h = host_scal(p);
+      const int nr = (int)h[S_CGN];
+      q.pol_cgr += nr;
+      q.pol_rounds += nr;
+      if (h[S_CGACT] == 0.0 || q.pol_cgr >= q.pol.cg_max) {
+        q.pol_incg = false;
+        Qstep.push_back(p);
+      }
+    }
+    if (!A2.empty()) {
+      dL(A2, X, G);    // pqdts.f90:2093
+      twometric(A2);   // pqdts.f90:2095-2102
+    }
+  }
+  // polish: the full Newton step and its largest entry
+  if (!Qstep.empty()) {
+    Scope sc(this, "polish");
+    launch_reduce(Qstep, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kb_pdas_step<<<tot, BLOCK, 0, s>>>(dprobs, wl, R_C);
+    }, R_C, S_STEP, true);
   }
   if (!A1.empty()) axpby(A1, SN, G, -1, -1.0, 0.0, AX_SCALE, false);  // pqdts.f90:1950
-  line_search(A);
-  // step, projection and re-evaluation, pqdts.f90:2111-2128
+  if (!A.empty()) line_search(A);
+  // step, projection and re-evaluation, pqdts.f90:2111-2128; a finished polish
+  // projects its result (clip + implicit column) and is evaluated the same way
   std::vector<int> stepS;
   for (int p : A)
     if (st[p]->alpha != 0.0) {
@@ -1156,11 +1288,15 @@ void Engine::tick() {
       stepS.push_back(p);
     }
   if (!stepS.empty()) axpby(stepS, X, SN, X, 0.0, 0.0, AX_AXPY, true);
-  project(A, X, false, false);
-  L(A, X, true);
-  dL(A, X, G);
-  pdot(A, G, G);
-  conv(A, X, G, false);
+  std::vector<int> AF = A;
+  AF.insert(AF.end(), Fz.begin(), Fz.end());
+  if (!AF.empty()) {
+    project(AF, X, false, false);
+    L(AF, X, true);
+    dL(AF, X, G);
+    pdot(AF, G, G);
+    conv(AF, X, G, false);
+  }
   fetch();
   for (int p : A) {
     PState &q = *st[p];
@@ -1170,6 +1306,24 @@ void Engine::tick() {
     q.emit(PQDTS_EV_ITER, {(double)q.stage, (double)q.iiter, Lval(p), std::fmin(0.0, h[S_XMIN]), q.conv,
                            std::sqrt(h[S_DOT]), (double)q.mode, q.alpha, q.dlalpha});
     if (q.conv < q.eps) q.stage_exit = true;
+  }
+  for (int p : Qstep) {
+    PState &q = *st[p];
+    q.pol_step = -host_scal(p)[S_STEP];
+    q.emit(PQDTS_EV_POLISH, {(double)q.pol_it, (double)q.pol_nact, (double)q.pol_nchg, (double)q.pol_cgr,
+                             q.pol_step});
+  }
+  for (int p : Fz) {
+    PState &q = *st[p];
+    q.conv = convval(p);
+    q.phase = 2;
+    pqdts_ctx::PolishOut &o = ctx[p]->pol_out;
+    o.status = q.pol_status;
+    o.iters = q.pol_it;
+    o.rounds = q.pol_rounds;
+    o.nact = q.pol_nact;
+    o.step = q.pol_step;
+    q.emit(PQDTS_EV_POLISHED, {(double)q.pol_status, (double)q.pol_it, (double)q.pol_rounds, q.conv, Lval(p)});
   }
 }
 
@@ -1782,6 +1936,8 @@ int engine_admit(Engine &e, pqdts_ctx *ctx, int mode, int maxiter, double eps, i
   q->output = output;
   q->cb = cb;
   q->user = user;
+  q->pol = ctx->pol;
+  ctx->pol_out = pqdts_ctx::PolishOut{};
   e.st[p].reset(q);
   ctx->full = true;
   return p;
@@ -2054,6 +2210,8 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
     q->output = output;
     q->cb = cb;
     q->user = user;
+    q->pol = ctx->pol;
+    ctx->pol_out = pqdts_ctx::PolishOut{};
     e.fp32[0] = 0;
     Engine::Scope sc(&e, "min");
     // the reference starts in full space (pqdts.f90:1225)
@@ -2067,6 +2225,35 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
     HIPC(hipStreamSynchronize(e.s));
     if (conv) *conv = q->conv;
     e.st[0].reset();
+  });
+}
+
+int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, int cg_max, int fp32) {
+  return guarded([&] {
+    checked(ctx);
+    require(max_outer >= 0 && cg_max >= 1 && cg_rtol > 0.0 && cg_rtol < 1.0 && tol > 0.0 &&
+                (fp32 == 0 || fp32 == 1),
+            "bad polish settings");
+    require(ctx->batch == nullptr, "the context is in a batch (release it first)");
+    ctx->pol.max_outer = max_outer;
+    ctx->pol.cg_rtol = cg_rtol;
+    ctx->pol.tol = tol;
+    ctx->pol.cg_max = cg_max;
+    ctx->pol.fp32 = fp32;
+  });
+}
+
+int pqdts_polish_result(pqdts_ctx *ctx, int *stats, int64_t *nact, double *step) {
+  return guarded([&] {
+    checked(ctx);
+    const pqdts_ctx::PolishOut &o = ctx->pol_out;
+    if (stats) {
+      stats[0] = o.status;
+      stats[1] = o.iters;
+      stats[2] = o.rounds;
+    }
+    if (nact) *nact = o.nact;
+    if (step) *step = o.step;
   });
 }
 

@@ -181,11 +181,9 @@ def random_problem(M: int, N: int, D: int, eta: float, gamma: float, shots: floa
                    {"eta": eta, "shots": shots, "seed": seed})
 
 
-def batch_mix(count: int = 256, seed: int = 4, m_lo: int = 50, m_hi: int = 2000,
-              eps: float = 1e-6, workers: int = 0) -> list[Problem]:
-    """Uneven instances: M log-uniform in [m_lo, m_hi], N uniform in [M/4, M], D = 2M,
-    eta uniform in [0.5, 0.95], gamma log-uniform in [1e-5, 1e-2], shots log-uniform
-    in [1e6, 1e8]."""
+def mix_specs(count: int = 256, seed: int = 4, m_lo: int = 50, m_hi: int = 2000,
+              eps: float = 1e-6) -> list[tuple]:
+    """The parameters of batch_mix's problems, in order (cheap: nothing is built)."""
     rng = np.random.default_rng(seed)
     specs = []
     for b in range(count):
@@ -195,6 +193,17 @@ def batch_mix(count: int = 256, seed: int = 4, m_lo: int = 50, m_hi: int = 2000,
         gamma = float(math.exp(rng.uniform(math.log(1e-5), math.log(1e-2))))
         shots = float(round(math.exp(rng.uniform(math.log(1e6), math.log(1e8)))))
         specs.append((M, N, 2 * M, eta, gamma, shots, seed * 100003 + b, eps, f"mix{b}"))
+    return specs
+
+
+def batch_mix(count: int = 256, seed: int = 4, m_lo: int = 50, m_hi: int = 2000,
+              eps: float = 1e-6, workers: int = 0, only=None) -> list[Problem]:
+    """Uneven instances: M log-uniform in [m_lo, m_hi], N uniform in [M/4, M], D = 2M,
+    eta uniform in [0.5, 0.95], gamma log-uniform in [1e-5, 1e-2], shots log-uniform
+    in [1e6, 1e8]. only: build just these indices of the mix (same problems)."""
+    specs = mix_specs(count, seed, m_lo, m_hi, eps)
+    if only is not None:
+        specs = [specs[i] for i in only]
     return build_many(specs, workers)
 
 

@@ -339,3 +339,62 @@ def test_oracle_mode1_matches_reference(cfg1):
         elif k == reftrace.EV_ITER:
             assert tuple(a[:2]) == tuple(b[:2]) and a[7] == b[7]
             assert _rel(a[2], b[2]) <= 1e-12 and _rel(a[4], b[4]) <= 1e-9
+
+
+# ---------------------------------------------------------------------------
+# oracle/qp.py: the exact minimiser and the KKT certificate the polish is
+# checked against
+# ---------------------------------------------------------------------------
+
+def test_qp_exact_matches_brute_force_enumeration():
+    """The dense active-set method equals the enumeration of every active set on
+    tiny problems (the textbook definition of the minimiser of a QP)."""
+    from oracle import qp
+    rng = np.random.default_rng(5)
+    for trial in range(6):
+        M, N, D = 3, 3, 5
+        F = rng.random((D, M))
+        P = rng.random((D, N))
+        P /= P.sum(1, keepdims=True)
+        gamma = 10 ** rng.uniform(-4, -1)
+        Xb = qp.brute_force(F, P, gamma)
+        Xe, ok = qp.exact(F, P, gamma)
+        assert ok and qp.is_minimiser(F, P, gamma, Xb), trial
+        assert np.max(np.abs(Xb - Xe)) <= 1e-12, trial
+
+
+def test_qp_objective_and_gradient_are_the_reference_operators(cfg1):
+    """qp's objective and gradient are the reference's L and dL (full space),
+    restated by the oracle: the certificate checks the reference's problem."""
+    from oracle import qp
+    o = oracle.Oracle.from_problem(cfg1)
+    rng = np.random.default_rng(2)
+    X = rng.random((cfg1.M, cfg1.N))
+    X /= X.sum(1, keepdims=True)
+    o.buf("X")[:] = X
+    o.set_fullspace(True)
+    L = o.L("X")
+    o.dL("X", "G")
+    assert abs(qp.objective(cfg1.F, cfg1.P, cfg1.gamma, X) - L) <= 1e-12 * abs(L)
+    g = qp.gradient(cfg1.F, cfg1.P, cfg1.gamma, X)
+    assert np.max(np.abs(g - o.buf("G"))) <= 1e-12 * np.max(np.abs(g))
+
+
+def test_qp_certificate_rejects_a_perturbed_minimiser():
+    """The exact minimiser of a 40 x 20 problem passes the KKT certificate; moving
+    1e-6 of probability between two free entries of one row, or onto an active
+    entry, fails it."""
+    from oracle import qp
+    p = workloads.random_problem(40, 20, 80, 0.8, 1e-3, 1e6, seed=3)
+    X, ok = qp.exact(p.F, p.P, p.gamma)
+    assert ok
+    free = np.argwhere(X[7] > 1e-3)[:, 0]
+    Y = X.copy()
+    Y[7, free[0]] += 1e-6
+    Y[7, free[1]] -= 1e-6
+    assert not qp.is_minimiser(p.F, p.P, p.gamma, Y)
+    zero = np.argwhere(X[7] == 0.0)[:, 0]
+    Y = X.copy()
+    Y[7, zero[0]] += 1e-6
+    Y[7, free[0]] -= 1e-6
+    assert not qp.is_minimiser(p.F, p.P, p.gamma, Y)
