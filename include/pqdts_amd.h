@@ -152,18 +152,20 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
  * active-set Newton iteration on the reduced problem (the implicit column of
  * each row is its largest entry): an entry is held at 0 when G - d X > 0 (G the
  * reduced gradient, d the reduced Hessian diagonal of its row); the free entries
- * take a full Newton step, solved by CG with the Jacobi preconditioner (at most
- * cg_max rounds per iteration) to the relative residual tol / (the last step's
- * largest entry), kept within [cg_rtol, 1e-3] (cg_rtol for the first step). It
- * stops when the active set repeats and the last solve's error estimate (its
- * relative residual times its step) is <= tol: a KKT point of the strictly
- * convex problem, i.e. its unique minimiser to about tol; or after max_outer
- * iterations. Gradient, active set, steps and the CG scalars are FP64; the
+ * take a full Newton step, solved by CG with the Jacobi preconditioner to the
+ * relative residual tol / (the last step's largest entry), kept within
+ * [cg_rtol, 1e-3] (cg_rtol for the first step). It stops when the active set
+ * repeats and the last solve's error estimate (its relative residual times its
+ * step) is <= tol: a KKT point of the strictly convex problem, i.e. its unique
+ * minimiser to about tol (status 1). It also stops after max_outer iterations
+ * or, with budget > 0, once its CG rounds reach budget times the PCG rounds of
+ * the minimisation itself (status 2); X is then the minimisation's result,
+ * unchanged. Gradient, active set, steps and the CG scalars are FP64; the
  * Hessian products are FP64 unless fp32 = 1 and the solve is mixed precision.
  * max_outer = 0 turns it off (the default). */
-int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, int cg_max, int fp32);
+int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, double budget, int fp32);
 /* outcome of the last polish on ctx: stats[0..2] = status (0 not run, 1 converged,
- * 2 stopped at max_outer), iterations, CG rounds; *nact active entries, *step
+ * 2 stopped at a cap), iterations, CG rounds; *nact active entries, *step
  * the last step's largest entry (each pointer may be NULL) */
 int pqdts_polish_result(pqdts_ctx *ctx, int *stats, int64_t *nact, double *step);
 
@@ -294,10 +296,13 @@ int pqdts_server_submit(pqdts_server *s, int64_t M, int64_t N, int64_t D, int64_
  * Returns the count, or -1. */
 int pqdts_server_wait(pqdts_server *s, int64_t *ids, int cap, int timeout_ms);
 /* result of a finished request, which is then forgotten: X (row-major M x N,
- * may be NULL), conv, stats[4] (finished, Newton iterations, PCG iterations,
- * line-search trials), times[3] seconds (queued, admitted to done, total).
+ * may be NULL), conv, stats[7] (finished, Newton iterations, PCG iterations,
+ * line-search trials, polish status / iterations / CG rounds as
+ * pqdts_polish_result), times[3] seconds (queued, admitted to done, total).
  * Returns the request's own status (nonzero if its solve failed). */
 int pqdts_server_result(pqdts_server *s, int64_t id, double *X, double *conv, int *stats, double *times);
+/* polish settings (as pqdts_set_polish) for the requests submitted from now on */
+int pqdts_server_set_polish(pqdts_server *s, int max_outer, double cg_rtol, double tol, double budget, int fp32);
 /* info[0..7] = queued, running, finished not yet waited for, device bytes held
  * now, peak device bytes held, budget, submitted, completed */
 int pqdts_server_info(pqdts_server *s, int64_t *info);

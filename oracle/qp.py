@@ -73,13 +73,11 @@ def is_minimiser(F, P, gamma, X, tol: float = 1e-9) -> bool:
     return r["x_min"] >= 0.0 and r["rowsum_err"] <= 1e-12 and r["stationarity"] <= tol and r["dual_min"] >= -tol
 
 
-def _face_solution(K, B, free):
-    """Minimiser of the quadratic on the face {X[~free] = 0, X 1 = 1}: per column
-    n, X[I_n, n] = K_II^-1 (B[I_n, n] - lambda[I_n]); the row sums fix lambda
-    through the Schur complement S = sum_n E_n K_II^-1 E_n^T."""
-    M, N = B.shape
+def _face_factors(K, free):
+    """Per column n, (I_n, K_II^-1) for the free rows I_n, and the Schur complement
+    S = sum_n E_n K_II^-1 E_n^T of the row-sum constraints."""
+    M, N = free.shape
     S = np.zeros((M, M))
-    rhs = -np.ones(M)
     inv = []
     for n in range(N):
         I = np.nonzero(free[:, n])[0]
@@ -88,13 +86,39 @@ def _face_solution(K, B, free):
             continue
         Ki = np.linalg.inv(K[np.ix_(I, I)])
         S[np.ix_(I, I)] += Ki
-        rhs[I] += Ki @ B[I, n]
         inv.append((I, Ki))
+    return inv, S
+
+
+def _face_solve(inv, S, B, c):
+    """x[I_n, n] = K_II^-1 (B[I_n, n] - lambda[I_n]) with the row sums of x equal to c."""
+    M, N = B.shape
+    rhs = -np.asarray(c, dtype=np.float64).copy()
+    for n, (I, Ki) in enumerate(inv):
+        if I.size:
+            rhs[I] += Ki @ B[I, n]
     lam = np.linalg.solve(S, rhs)
     X = np.zeros((M, N))
     for n, (I, Ki) in enumerate(inv):
         if I.size:
             X[I, n] = Ki @ (B[I, n] - lam[I])
+    return X, lam
+
+
+def _face_solution(K, B, free):
+    """Minimiser of the quadratic on the face {X[~free] = 0, X 1 = 1}: per column
+    n, X[I_n, n] = K_II^-1 (B[I_n, n] - lambda[I_n]); the row sums fix lambda
+    through the Schur complement. Two steps of iterative refinement on the
+    residuals of both equations recover the accuracy the explicit inverses of an
+    ill-conditioned K lose."""
+    inv, S = _face_factors(K, free)
+    M = B.shape[0]
+    X, lam = _face_solve(inv, S, B, np.ones(M))
+    for _ in range(2):
+        R = np.where(free, B - K @ X - lam[:, None], 0.0)
+        dX, dlam = _face_solve(inv, S, R, 1.0 - X.sum(1))
+        X += dX
+        lam += dlam
     return X, lam
 
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "pqdts_device_cu_count", "pqdts_batch_set_cu_mask", "simplexproj_condat_",
     "pqdts_simplexproj_rows_noalloc", "pqdts_server_create", "pqdts_server_destroy", "pqdts_server_submit",
     "pqdts_server_wait", "pqdts_server_result", "pqdts_server_info", "pqdts_footprint", "pqdts_device_timers",
-    "pqdts_set_polish", "pqdts_polish_result",
+    "pqdts_set_polish", "pqdts_polish_result", "pqdts_server_set_polish",
 ]
 
 _lib = None
@@ -104,8 +104,9 @@ def load():
     L.pqdts_info.argtypes = [vp, C.POINTER(i64)]
     L.pqdts_set_hvp_path.argtypes = [vp, i32]
     L.pqdts_set_hvp_split.argtypes = [vp, i32]
-    L.pqdts_set_polish.argtypes = [vp, i32, d, d, i32, i32]
+    L.pqdts_set_polish.argtypes = [vp, i32, d, d, d, i32]
     L.pqdts_polish_result.argtypes = [vp, C.POINTER(i32), i64p, dp]
+    L.pqdts_server_set_polish.argtypes = [vp, i32, d, d, d, i32]
     L.pqdts_batch_create.argtypes = [i32, C.POINTER(vp)]
     L.pqdts_batch_destroy.argtypes = [vp]
     L.pqdts_batch_admit.argtypes = [vp, vp, i32, i32, d, i32, i32, i32, EVENT_CB, vp, C.POINTER(i32)]
@@ -343,10 +344,11 @@ class Context:
         """Split of the Gram product over band chunks: 0 by shape, 1 none, 2..16 fixed."""
         _check(load().pqdts_set_hvp_split(self.h, int(split)))
 
-    def set_polish(self, max_outer=100, cg_rtol=1e-6, tol=1e-10, cg_max=20000, fp32=False):
+    def set_polish(self, max_outer=100, cg_rtol=1e-6, tol=1e-10, budget=0.0, fp32=False):
         """Polish every later solve of this context to the problem's minimiser (primal-dual
-        active-set Newton after conv < eps) to about `tol`; max_outer = 0 turns it off."""
-        _check(load().pqdts_set_polish(self.h, int(max_outer), float(cg_rtol), float(tol), int(cg_max),
+        active-set Newton after conv < eps) to about `tol`, with at most `budget` times the
+        minimisation's PCG rounds (0: no limit); max_outer = 0 turns it off."""
+        _check(load().pqdts_set_polish(self.h, int(max_outer), float(cg_rtol), float(tol), float(budget),
                                        int(bool(fp32))))
 
     def polish_result(self) -> dict:
@@ -553,11 +555,17 @@ class Server:
         M, N = self.shape.pop(rid)
         X = np.empty((M, N))
         conv = C.c_double()
-        stats = (C.c_int * 4)()
+        stats = (C.c_int * 7)()
         times = (C.c_double * 3)()
         _check(load().pqdts_server_result(self.h, int(rid), _p(X), C.byref(conv), stats, times))
         return {"X": X, "conv": conv.value, "finished": bool(stats[0]), "newton": stats[1], "pcg": stats[2],
-                "trials": stats[3], "queued_s": times[0], "run_s": times[1], "latency_s": times[2]}
+                "trials": stats[3], "queued_s": times[0], "run_s": times[1], "latency_s": times[2],
+                "polish": {"status": stats[4], "iterations": stats[5], "cg_rounds": stats[6]}}
+
+    def set_polish(self, max_outer=100, cg_rtol=1e-6, tol=1e-10, budget=0.0, fp32=False):
+        """Polish settings (Context.set_polish) for requests submitted from now on."""
+        _check(load().pqdts_server_set_polish(self.h, int(max_outer), float(cg_rtol), float(tol), float(budget),
+                                              int(bool(fp32))))
 
     def info(self) -> dict:
         a = (C.c_int64 * 8)()

@@ -166,11 +166,13 @@ struct pqdts_ctx {
     int max_outer = 0;        // 0: no polish
     double cg_rtol = 1e-6;    // the tightest relative CG residual
     double tol = 1e-10;       // target distance of the result from the minimiser
-    int cg_max = 20000;       // CG rounds of one outer iteration, at most
+    double budget = 0.0;      // CG rounds of the whole polish, at most this multiple of the
+                              // minimisation's PCG rounds (<= 0: no limit)
     int fp32 = 0;             // 1: FP32 Hessian products where the solve is mixed
   } pol;
   struct PolishOut {
-    int status = 0;           // 0 not run, 1 converged, 2 stopped at the iteration cap
+    int status = 0;           // 0 not run, 1 converged, 2 stopped at a cap (result: the
+                              // minimisation's own)
     int iters = 0, rounds = 0;
     int64_t nact = 0;
     double step = 0.0;
@@ -930,6 +932,7 @@ struct PState {
   pqdts_ctx::Polish pol;
   int pol_it = 0;         // outer iterations started
   int pol_cgr = 0;        // CG rounds of the current outer iteration
+  int64_t pol_budget = 0; // CG rounds the polish may use (0: no limit)
   int pol_rounds = 0;     // CG rounds of all outer iterations
   bool pol_incg = false;  // the current outer iteration's CG continues next step
   bool pol_entered = false;
@@ -1140,11 +1143,17 @@ void Engine::tick() {
     (q.pol_incg ? Qc : Qs).push_back(p);
     if (!q.pol_entered) {
       q.pol_entered = true;
-      ctx[p]->full = false;
+      q.pol_budget = q.pol.budget > 0.0 ? std::max<int64_t>(1, (int64_t)(q.pol.budget * q.pcg)) : 0;
       Qent.push_back(p);
     }
   }
-  if (!Qent.empty()) transform(Qent, true);
+  if (!Qent.empty()) {
+    // the minimisation's result (full space) is kept in Z: the result if the polish
+    // stops at a cap
+    axpby(Qent, PQDTS_Z, X, -1, 1.0, 0.0, AX_COPY, false);
+    for (int p : Qent) ctx[p]->full = false;
+    transform(Qent, true);
+  }
   std::vector<int> AQ = A;
   AQ.insert(AQ.end(), Qs.begin(), Qs.end());
   if (!AQ.empty()) dL(AQ, X, G);
@@ -1167,7 +1176,7 @@ void Engine::tick() {
       if (q.pol_it > 0 && q.pol_nchg == 0 && q.pol_rtol * q.pol_step <= q.pol.tol) {
         q.pol_status = 1;
         Fz.push_back(p);
-      } else if (q.pol_it >= q.pol.max_outer) {
+      } else if (q.pol_it >= q.pol.max_outer || (q.pol_budget > 0 && q.pol_rounds >= q.pol_budget)) {
         q.pol_status = 2;
         Fz.push_back(p);
       } else {
@@ -1260,7 +1269,7 @@ void Engine::tick() {
       const int nr = (int)h[S_CGN];
       q.pol_cgr += nr;
       q.pol_rounds += nr;
-      if (h[S_CGACT] == 0.0 || q.pol_cgr >= q.pol.cg_max) {
+      if (h[S_CGACT] == 0.0 || (q.pol_budget > 0 && q.pol_rounds >= q.pol_budget)) {
         q.pol_incg = false;
         Qstep.push_back(p);
       }
@@ -1288,10 +1297,16 @@ void Engine::tick() {
       stepS.push_back(p);
     }
   if (!stepS.empty()) axpby(stepS, X, SN, X, 0.0, 0.0, AX_AXPY, true);
-  std::vector<int> AF = A;
-  AF.insert(AF.end(), Fz.begin(), Fz.end());
+  // a polish stopped at a cap returns the minimisation's result unchanged
+  std::vector<int> Fr, AF = A;
+  for (int p : Fz) (st[p]->pol_status == 1 ? AF : Fr).push_back(p);
+  if (!AF.empty()) project(AF, X, false, false);
+  if (!Fr.empty()) {
+    axpby(Fr, X, PQDTS_Z, -1, 1.0, 0.0, AX_COPY, false);
+    for (int p : Fr) ctx[p]->full = true;
+    AF.insert(AF.end(), Fr.begin(), Fr.end());
+  }
   if (!AF.empty()) {
-    project(AF, X, false, false);
     L(AF, X, true);
     dL(AF, X, G);
     pdot(AF, G, G);
@@ -2228,17 +2243,16 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
   });
 }
 
-int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, int cg_max, int fp32) {
+int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, double budget, int fp32) {
   return guarded([&] {
     checked(ctx);
-    require(max_outer >= 0 && cg_max >= 1 && cg_rtol > 0.0 && cg_rtol < 1.0 && tol > 0.0 &&
-                (fp32 == 0 || fp32 == 1),
+    require(max_outer >= 0 && cg_rtol > 0.0 && cg_rtol < 1.0 && tol > 0.0 && (fp32 == 0 || fp32 == 1),
             "bad polish settings");
     require(ctx->batch == nullptr, "the context is in a batch (release it first)");
     ctx->pol.max_outer = max_outer;
     ctx->pol.cg_rtol = cg_rtol;
     ctx->pol.tol = tol;
-    ctx->pol.cg_max = cg_max;
+    ctx->pol.budget = budget;
     ctx->pol.fp32 = fp32;
   });
 }

@@ -34,7 +34,8 @@ struct SrvRequest {
   std::string err;
   std::vector<double> X;  // row-major result
   double conv = 0.0;
-  int stats[4] = {0, 0, 0, 0};
+  int stats[7] = {0, 0, 0, 0, 0, 0, 0};  // finished, Newton, PCG, trials, polish status, iterations, rounds
+  pqdts_ctx::Polish pol;                 // the server's polish settings when submitted
 };
 
 }  // namespace
@@ -51,6 +52,7 @@ struct pqdts_server {
   std::vector<std::thread> workers;
   bool stop = false;
   int device = 0;
+  pqdts_ctx::Polish pol;  // applied to requests submitted from now on
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -116,6 +118,7 @@ void server_lane(pqdts_server *S, int g) {
       const int rc = guarded([&] {
         c = create_ctx(r->M, r->N, r->D, (int64_t)r->Fv.size(), r->Fr.data(), r->Fc.data(), r->Fv.data(),
                        (int64_t)r->Pv.size(), r->Pr.data(), r->Pc.data(), r->Pv.data(), r->gamma);
+        c->pol = r->pol;
         if (r->X0.empty()) r->X0.assign((size_t)r->M * r->N, 1.0 / (double)r->N);
         copy_slot(c, PQDTS_X, nullptr, r->X0.data(), PQDTS_ROW_MAJOR, true);
         const int slot = engine_admit(e, c, r->mode, r->maxiter, r->eps, 1, 0, r->precision, nullptr, nullptr);
@@ -158,6 +161,9 @@ void server_lane(pqdts_server *S, int g) {
       r->stats[1] = q.newton;
       r->stats[2] = q.pcg;
       r->stats[3] = q.trials;
+      r->stats[4] = c->pol_out.status;
+      r->stats[5] = c->pol_out.iters;
+      r->stats[6] = c->pol_out.rounds;
       std::string msg;
       const int rc2 = guarded([&] {
         r->X.resize((size_t)r->M * r->N);
@@ -288,6 +294,7 @@ int pqdts_server_submit(pqdts_server *s, int64_t M, int64_t N, int64_t D, int64_
     std::lock_guard<std::mutex> lk(s->mu);
     require(!s->stop, "server stopped");
     r->id = s->next_id++;
+    r->pol = s->pol;
     r->lane = lane_of(s, M * N);
     r->t_submit = s->now();
     if (id) *id = r->id;
@@ -335,7 +342,7 @@ int pqdts_server_result(pqdts_server *s, int64_t id, double *X, double *conv, in
     if (X && !r->X.empty()) std::memcpy(X, r->X.data(), sizeof(double) * r->X.size());
     if (conv) *conv = r->conv;
     if (stats)
-      for (int k = 0; k < 4; k++) stats[k] = r->stats[k];
+      for (int k = 0; k < 7; k++) stats[k] = r->stats[k];
     if (times) {
       times[0] = r->t_admit - r->t_submit;
       times[1] = r->t_done - r->t_admit;
@@ -345,6 +352,20 @@ int pqdts_server_result(pqdts_server *s, int64_t id, double *X, double *conv, in
     if (status != 0) throw Fail(status, "request " + std::to_string(id) + " failed: " + r->err);
   });
   return rc;
+}
+
+int pqdts_server_set_polish(pqdts_server *s, int max_outer, double cg_rtol, double tol, double budget, int fp32) {
+  return guarded([&] {
+    require(s != nullptr, "null server");
+    require(max_outer >= 0 && cg_rtol > 0.0 && cg_rtol < 1.0 && tol > 0.0 && (fp32 == 0 || fp32 == 1),
+            "bad polish settings");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->pol.max_outer = max_outer;
+    s->pol.cg_rtol = cg_rtol;
+    s->pol.tol = tol;
+    s->pol.budget = budget;
+    s->pol.fp32 = fp32;
+  });
 }
 
 int pqdts_server_info(pqdts_server *s, int64_t *info) {

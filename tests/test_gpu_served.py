@@ -100,3 +100,38 @@ def test_served_stream_matches_alone_and_respects_budget(stream):
         assert r["finished"]
         assert np.array_equal(r["X"], Xa), f"request {k} ({p.name}, precision {prec})"
         assert r["conv"] == conva
+
+
+def test_served_polish_matches_alone(stream):
+    """Polished requests (server-wide polish settings, FP64 and mixed with FP32
+    polish products) return the bits of the same polished solve alone, and the
+    polish outcome with them."""
+    small, _ = stream
+    reqs = [(small[k], prec) for k in (1, 3, 6) for prec in (0, 1)]
+    srv = pqdts_amd.Server(BOUNDS, capacity=8)
+    srv.set_polish(40, fp32=True)
+    ids = {srv.submit(p, maxiter=300, precision=prec): k for k, (p, prec) in enumerate(reqs)}
+    out = {}
+    t0 = time.time()
+    while len(out) < len(reqs):
+        for rid in srv.wait(timeout_ms=1000):
+            out[ids[rid]] = srv.result(rid)
+        assert time.time() - t0 < 300
+    srv.close()
+    for k, (p, prec) in enumerate(reqs):
+        c = pqdts_amd.Context.from_problem(p)
+        c.set_polish(40, fp32=True)
+        c.pset(1.0 / p.N, "X")
+        if prec == 0:
+            c.solve(2, 300, p.eps, 1, 0)
+        else:
+            b = pqdts_amd.Batch(1)
+            b.admit(c, 2, 300, p.eps, 1, 0, 1, record=False)
+            b.run()
+            b.release(0)
+            b.close()
+        pol = c.polish_result()
+        assert np.array_equal(out[k]["X"], c.get("X")), (p.name, prec)
+        assert out[k]["polish"] == {"status": pol["status"], "iterations": pol["iterations"],
+                                    "cg_rounds": pol["cg_rounds"]}, (p.name, prec)
+        c.close()
