@@ -208,16 +208,24 @@ def cpu_baseline_cfg2(prob, op_counts, threads):
 # ------------------------------------------------------------------ workloads
 HVP_SPLIT = -1  # --hvp-split: -1 leaves the library's choice
 POLISH = 0      # --polish: outer iterations of the polish after conv < eps (0: none)
+POLISH_CLASSES = 2  # --polish-classes
 POLISH_BUDGET = 1.0  # --polish-budget: polish CG rounds, at most this multiple of the solve's PCG rounds
 
 
-def make_ctx(pqdts_amd, prob):
+def make_ctx(pqdts_amd, prob, polish=True):
     c = pqdts_amd.Context.from_problem(prob)
     if HVP_SPLIT >= 0:
         c.set_hvp_split(HVP_SPLIT)
-    if POLISH > 0:
+    if POLISH > 0 and polish:
         c.set_polish(POLISH, budget=POLISH_BUDGET, fp32=True)
     return c
+
+
+def polish_set(problems, classes):
+    """Indices of the problems in the `classes` smallest of the lanes' three size classes
+    (thirds by M x N): the ones the polish is applied to."""
+    order = sorted(range(len(problems)), key=lambda i: problems[i].M * problems[i].N)
+    return set(order[:len(order) * max(0, min(3, classes)) // 3])
 
 
 def lane_groups(problems, lanes):
@@ -434,11 +442,20 @@ def certified(r):
 def polish_summary(res_m, res_b64, problems):
     if POLISH == 0:
         return None
-    out = {"max_outer": POLISH, "budget": POLISH_BUDGET,
-           "not_converged": [{"M": problems[i].M, "N": problems[i].N, "gamma": problems[i].gamma,
-                              "mixed": res_m[i]["polish"], "fp64": res_b64[i]["polish"]}
-                             for i in range(len(problems))
-                             if res_m[i]["polish"]["status"] != 1 or res_b64[i]["polish"]["status"] != 1][:12]}
+    out = {"max_outer": POLISH, "budget": POLISH_BUDGET, "size_classes_polished": POLISH_CLASSES,
+           "not_polished": sum(1 for r in res_m if r["polish"]["status"] == 0),
+           "stopped_uncertified": [{"M": problems[i].M, "N": problems[i].N, "gamma": problems[i].gamma,
+                                    "mixed": res_m[i]["polish"], "fp64": res_b64[i]["polish"]}
+                                   for i in range(len(problems))
+                                   if 2 in (res_m[i]["polish"]["status"], res_b64[i]["polish"]["status"])][:12]}
+    order = sorted(range(len(problems)), key=lambda i: problems[i].M * problems[i].N)
+    thirds = [order[len(order) * g // 3: len(order) * (g + 1) // 3] for g in range(3)]
+    out["certified_by_size_third"] = {
+        tag: [sum(1 for i in t if res[i]["polish"]["status"] == 1) for t in thirds]
+        for tag, res in (("mixed", res_m), ("fp64", res_b64))}
+    out["cg_rounds_by_size_third"] = {
+        tag: [int(sum(res[i]["polish"]["cg_rounds"] for i in t)) for t in thirds]
+        for tag, res in (("mixed", res_m), ("fp64", res_b64))}
     for tag, res in (("mixed", res_m), ("fp64", res_b64)):
         pr = [r["polish"] for r in res]
         out[tag] = {"certified": sum(1 for q in pr if q["status"] == 1), "of": len(pr),
@@ -457,7 +474,8 @@ def workload_mix(args, pqdts_amd, workloads):
     if pqdts_amd.device_count() < 1:
         raise SystemExit("no GPU visible")
     t = time.time()
-    ctxs = [make_ctx(pqdts_amd, p) for p in problems]
+    pol_idx = polish_set(problems, args.polish_classes)
+    ctxs = [make_ctx(pqdts_amd, p, i in pol_idx) for i, p in enumerate(problems)]
     n_gram = sum(1 for c in ctxs if c.info()["gram"])
     log(f"uploaded in {time.time() - t:.1f} s; {n_gram} problems take the Gram Hessian product")
     lane_cus = parse_lane_cus(args, pqdts_amd)
@@ -790,16 +808,24 @@ def workload_served(args, pqdts_amd, workloads):
     # every lane shares every CU unless ranges are given
     lane_cus = None if args.lane_cus.strip() == "auto" else parse_lane_cus(args, pqdts_amd)
 
+    def polished_req(i):
+        """The polish applies to the requests of the size classes below the largest one
+        (as in the mix bench): the largest problems are not certified within its budget."""
+        return reqs[i].M * reqs[i].N < bounds[min(len(bounds), POLISH_CLASSES) - 1] if POLISH_CLASSES < 3 else True
+
     def serve(precision):
         srv = pqdts_amd.Server(bounds, lane_cus=lane_cus, capacity=args.capacity or 256, budget_bytes=budget)
-        if POLISH > 0:
-            srv.set_polish(POLISH, budget=POLISH_BUDGET, fp32=True)
         lat, X, conv, cert, ids = [None] * n, [None] * n, [None] * n, [None] * n, {}
+        pol_now = None
         t0 = time.perf_counter()
         nxt = 0
         while nxt < n or len(ids):
             now = time.perf_counter() - t0
             while nxt < n and arrivals[nxt] <= now:
+                if POLISH > 0 and polished_req(nxt) != pol_now:
+                    # the server applies its polish settings to the requests submitted after them
+                    pol_now = polished_req(nxt)
+                    srv.set_polish(POLISH if pol_now else 0, budget=POLISH_BUDGET, fp32=True)
                 ids[srv.submit(reqs[nxt], maxiter=args.maxiter, precision=precision, coo=coos[nxt])] = nxt
                 nxt += 1
             wait_ms = int(max(1.0, 1e3 * (arrivals[nxt] - now))) if nxt < n else 1000
@@ -823,7 +849,7 @@ def workload_served(args, pqdts_amd, workloads):
                 time.sleep(arrivals[i] - now)
             (fr, fc, fv), (pr, pc, pv) = coos[i]
             c = pqdts_amd.Context(reqs[i].M, reqs[i].N, reqs[i].D, (fr, fc, fv), (pr, pc, pv), reqs[i].gamma)
-            if POLISH > 0:
+            if POLISH > 0 and polished_req(i):
                 c.set_polish(POLISH, budget=POLISH_BUDGET)
             c.pset(1.0 / reqs[i].N, "X")
             c.solve(2, args.maxiter, reqs[i].eps, 1, 0)
@@ -919,16 +945,20 @@ def main():
     ap.add_argument("--polish", type=int, default=60,
                     help="polish every solve to the problem's minimiser after conv < eps, at most this many "
                          "primal-dual active-set iterations (0: off)")
+    ap.add_argument("--polish-classes", type=int, default=2,
+                    help="polish the problems of this many of the three size classes (thirds by M x N), "
+                         "smallest first; measured: the largest third is not certified within the budget")
     ap.add_argument("--polish-budget", type=float, default=1.0,
                     help="polish CG rounds, at most this multiple of the minimisation's own PCG rounds")
     ap.add_argument("--hvp-split", type=int, default=-1,
                     help="split of the Gram product over band chunks (0 by shape, 1 none, k fixed)")
     args = ap.parse_args()
-    global HVP_SPLIT, POLISH, POLISH_BUDGET
+    global HVP_SPLIT, POLISH, POLISH_BUDGET, POLISH_CLASSES
     HVP_SPLIT = args.hvp_split
     # the polish applies to the mix (the metric's 1e-6 gate); the other workloads report
     # the reference's own stopping point
     POLISH, POLISH_BUDGET = (args.polish if args.workload in ("mix", "served") else 0), args.polish_budget
+    POLISH_CLASSES = args.polish_classes
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); independent problems would "
               f"shard one per GPU with no communication, so N GPUs serve N times the problems of one",
