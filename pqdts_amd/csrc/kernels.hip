@@ -1513,7 +1513,9 @@ __global__ __launch_bounds__(BLOCK) void kb_cg_update(const Prob *__restrict__ p
   ROW_PROLOGUE();
   const double v2 = reduce_parts_all(gp(P.part) + (int64_t)i_reg_v2 * P.pcap, gp(P.pn)[i_reg_v2], sh, &bc);
   if (threadIdx.x == 0 && lb == 0) gp(P.scal)[i_v2] = v2;
-  const double a = gp(P.scal)[i_b1] / v2;
+  // a polish face without free entries (or with a zero gradient on them) has
+  // r = p = 0 and v2 = 0: no step instead of 0/0
+  const double a = (polish && !(v2 > 0.0)) ? 0.0 : gp(P.scal)[i_b1] / v2;
   double rz = 0.0, rr = 0.0;
   if (j < P.M) cg_update_row(P, j, full, late && !full, polish, a, rz, rr);
   double t = block_sum(rz, sh);

@@ -95,3 +95,21 @@ def test_polish_removes_the_summation_order_spread(probs):
     for p, a, b in zip(sub, rg, rf):
         assert a[2]["status"] == 1 and b[2]["status"] == 1, p.name
         assert np.max(np.abs(a[0] - b[0])) <= 1e-8, p.name
+
+
+def test_polish_at_a_vertex_minimiser():
+    """A problem whose minimiser is a vertex of every row's simplex (noise-free
+    P = F Pi_true with one-hot rows, gamma = 0, F of full column rank): the last
+    faces have no free entries (r = p = 0, p.Hp = 0). The polish ends at Pi_true
+    with no 0/0 in its CG step, in both precisions."""
+    import scipy.sparse as sps
+    rng = np.random.default_rng(8)
+    M, N, D = 12, 5, 30
+    F = sps.csr_matrix(rng.random((D, M)) + 0.1)
+    pi = np.zeros((M, N))
+    pi[np.arange(M), rng.integers(0, N, M)] = 1.0
+    p = workloads.Problem("vertex", M, N, D, F, np.ascontiguousarray(F @ pi), 0.0, 1e-9, pi)
+    for prec in (0, 1):
+        (X, res, pol, _), = run_batch([p], prec)
+        assert pol["status"] == 1 and np.all(np.isfinite(X)), (prec, pol)
+        assert np.max(np.abs(X - pi)) <= 1e-9, prec
