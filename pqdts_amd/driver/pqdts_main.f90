@@ -9,13 +9,20 @@
 ! through iso_c_binding. stdout uses explicit formats that reproduce the
 ! fields pqdts.py parses (a line starting with " iter   " whose fifth
 ! whitespace token is the objective, pqdts.py:77-78).
+!
+! PQDTS_POLISH=k (environment; no counterpart in the reference) continues the
+! solve after the reference's stopping test with at most k polish iterations
+! to the problem's minimiser (pqdts_set_polish). The per-stage files keep the
+! reference's stopping point; rank_*_final.dat holds the polished result. Its
+! progress lines start with " polish", which pqdts.py's parse ignores.
 module pqdts_capi
   use iso_c_binding
   implicit none
   integer(c_int), parameter :: SLOT_X = 0, SLOT_G = 1, SLOT_HP = 2, SLOT_XTMP = 3, SLOT_SN = 4, SLOT_Z = 5
   integer(c_int), parameter :: COL_MAJOR = 1
   integer(c_int), parameter :: EV_ITER = 0, EV_PCG = 1, EV_LS = 2, EV_ACTIVE = 3, EV_STOP1 = 4, &
-                               EV_ITER0 = 5, EV_LAMBDA = 6, EV_STAGE = 7
+                               EV_ITER0 = 5, EV_LAMBDA = 6, EV_STAGE = 7, EV_POLISH = 8, &
+                               EV_POLISHED = 9
   interface
     function pqdts_last_error() bind(C, name="pqdts_last_error")
       import :: c_ptr
@@ -136,6 +143,13 @@ module pqdts_capi
       type(c_funptr), value :: cb
       type(c_ptr), value :: user
       real(c_double) :: conv
+    end function
+    integer(c_int) function pqdts_set_polish(ctx, max_outer, cg_rtol, tol, budget, fp32) &
+      bind(C, name="pqdts_set_polish")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: max_outer, fp32
+      real(c_double), value :: cg_rtol, tol, budget
     end function
     integer(c_int) function pqdts_device_timers(ctx, on) bind(C, name="pqdts_device_timers")
       import :: c_int, c_ptr
@@ -329,6 +343,12 @@ contains
       tlast = now
     case (EV_LAMBDA)
       print '(a,i12,2es25.16e3)', " lambda", nint(v(1)), v(2), v(3)
+    case (EV_POLISH)
+      print '(a,i12,a,i12,a,i12,a,i12,a,es25.16e3)', " polish", nint(v(1)), " active ", nint(v(2), int64), &
+        " changed ", nint(v(3), int64), " cg ", nint(v(4)), " step ", v(5)
+    case (EV_POLISHED)
+      print '(a,i3,a,i12,a,i12,a,es25.16e3,a,es25.16e3)', " polished status ", nint(v(1)), " iterations ", &
+        nint(v(2)), " cg ", nint(v(3)), " conv ", v(4), " L= ", v(5)
     case (EV_STAGE)
       if (output >= 2) then
         write (fname, '(a,i6,a,i6,a)') "rank_", rank, "_oiter", nint(v(1)), ".dat"
@@ -421,6 +441,12 @@ program pqdts_amd_driver
   call check(pqdts_create(M, N, D, nnzF, Fr, Fc, Fv, nnzP, Pr, Pc, Pv, gam, ctx), "pqdts_create")
   ! the solver's timers measure device time, as the reference's measure its own work
   call check(pqdts_device_timers(ctx, 1_c_int), "pqdts_device_timers")
+  call get_environment_variable("PQDTS_POLISH", arg, status=it)
+  if (it == 0) then
+    read (arg, *) it
+    if (it > 0) call check(pqdts_set_polish(ctx, int(it, c_int), 1e-6_c_double, 1e-10_c_double, &
+                                             0.0_c_double, 0_c_int), "pqdts_set_polish")
+  end if
   deallocate (Fr, Fc, Fv, Pr, Pc, Pv)
   allocate (X(M, N))
 

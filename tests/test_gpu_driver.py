@@ -177,3 +177,27 @@ def test_driver_matches_reference_case(tmp_path, name):
             assert v[0] == w[0] and _close(v[1], w[1], 1e-6), (name, v, w)
     _, _, _, _, _, X = fortio.read_state(str(tmp_path / "rank_     0_final.dat"))
     assert np.max(np.abs(X - states[name])) <= 1e-9, (name, np.max(np.abs(X - states[name])))
+
+
+def test_driver_polish_env(tmp_path, cfg1):
+    """PQDTS_POLISH=k: the per-stage files and every " iter " line keep the reference's
+    stopping point (the same bits as without it); rank_*_final.dat holds the polished
+    result: within 1e-8 of the point the oracle's exact active-set method certifies
+    as the problem's minimiser (oracle/qp.py)."""
+    from oracle import qp
+    NF, NP = workloads.write_inputs(str(tmp_path), cfg1)
+    plain = run(str(tmp_path), cfg1, NF, NP, maxiter=200, output=2)
+    X2_plain = fortio.read_state(str(tmp_path / "rank_     0_oiter     2.dat"))[5]
+    os.environ["PQDTS_POLISH"] = "40"
+    try:
+        out = run(str(tmp_path), cfg1, NF, NP, maxiter=200, output=2)
+    finally:
+        del os.environ["PQDTS_POLISH"]
+    X2 = fortio.read_state(str(tmp_path / "rank_     0_oiter     2.dat"))[5]
+    assert np.array_equal(X2, X2_plain)
+    strip = lambda s: [ln.split(" dt= ")[0] for ln in s.splitlines() if ln.startswith(" iter   ")]
+    assert strip(out) == strip(plain)
+    assert any(ln.startswith(" polished status   1") for ln in out.splitlines())
+    X = pqdts_py_final(str(tmp_path))[5]
+    Xe, ok = qp.exact(cfg1.F, cfg1.P, cfg1.gamma, X0=X)
+    assert ok and np.max(np.abs(X - Xe)) <= 1e-8
