@@ -28,3 +28,26 @@ def test_lane_cus_forms():
 def test_lane_cus_rejects(spec):
     with pytest.raises(SystemExit):
         parse(spec)
+
+
+def test_polish_set_takes_the_smaller_size_classes():
+    """--polish-classes k polishes the problems of the k smallest of the three size
+    classes (thirds by M x N), the lanes' own split."""
+    probs = [types.SimpleNamespace(M=m, N=n) for m, n in [(50, 20), (2000, 1000), (100, 90), (700, 300),
+                                                          (60, 60), (1500, 400)]]
+    assert bench.polish_set(probs, 0) == set()
+    assert bench.polish_set(probs, 1) == {0, 4}
+    assert bench.polish_set(probs, 2) == {0, 4, 2, 3}
+    assert bench.polish_set(probs, 3) == set(range(6))
+
+
+def test_certified_reads_the_polish_status():
+    bench.POLISH = 60
+    try:
+        assert bench.certified({"polish": {"status": 1}})
+        assert not bench.certified({"polish": {"status": 2}})
+        assert not bench.certified({"polish": {"status": 0}})
+        assert not bench.certified({})
+    finally:
+        bench.POLISH = 0
+    assert bench.certified({})

@@ -113,3 +113,27 @@ def test_polish_at_a_vertex_minimiser():
         (X, res, pol, _), = run_batch([p], prec)
         assert pol["status"] == 1 and np.all(np.isfinite(X)), (prec, pol)
         assert np.max(np.abs(X - pi)) <= 1e-9, prec
+
+
+def test_polish_stopped_by_its_budget_returns_the_minimisation_result(probs):
+    """A polish that reaches its CG budget (here 0.05 x the minimisation's own PCG
+    rounds) reports status 2 and leaves X and conv exactly as the solve without a
+    polish ends them, in both precisions."""
+    p = probs[5]  # M = 229: its polish needs more rounds than the budget allows
+    for prec in (0, 1):
+        outs = []
+        for budget in (None, 0.05):
+            c = pqdts_amd.Context.from_problem(p)
+            if budget is not None:
+                c.set_polish(60, budget=budget)
+            c.put("X", np.full((p.M, p.N), 1.0 / p.N))
+            b = pqdts_amd.Batch(1)
+            b.admit(c, 2, 5000, EPS, 1, 0, prec, record=False)
+            res = b.run()[0]
+            b.release(0)
+            b.close()
+            outs.append((c.get("X"), res["conv"], c.polish_result()))
+            c.close()
+        (X0, conv0, pol0), (X1, conv1, pol1) = outs
+        assert pol0["status"] == 0 and pol1["status"] == 2, (prec, pol1)
+        assert np.array_equal(X0, X1) and conv0 == conv1, prec
