@@ -155,9 +155,9 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
  * take a full Newton step, solved by CG with the Jacobi preconditioner to the
  * relative residual tol / (the last step's largest entry), kept within
  * [cg_rtol, 1e-3] (cg_rtol for the first step). It stops when the active set
- * repeats and the last solve's error estimate (its relative residual times its
- * step) is <= tol: a KKT point of the strictly convex problem, i.e. its unique
- * minimiser to about tol (status 1). It also stops after max_outer iterations
+ * repeats, the last step is <= 100 tol and that solve's error estimate (its
+ * relative residual times its step) is <= tol: a KKT point of the strictly
+ * convex problem, i.e. its unique minimiser to about tol (status 1). It also stops after max_outer iterations
  * or, with budget > 0, once its CG rounds reach budget times the PCG rounds of
  * the minimisation itself (status 2); X is then the minimisation's result,
  * unchanged. Gradient, active set, steps and the CG scalars are FP64; the

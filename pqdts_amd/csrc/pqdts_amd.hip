@@ -252,7 +252,9 @@ struct Engine {
   // (the tail of a batch repeats one set for thousands of Newton iterations).
   // Batched engines only, without kernel timing or synchronising timers.
   std::map<std::vector<int64_t>, hipGraphExec_t> cg_graphs;
-  bool graphs_on = false;
+  // 0: eager launches; 1: CG rounds in captured blocks of 8 between host checks;
+  // 2: a Newton iteration as two captured graphs around the host-driven line search
+  int graphs_on = 0;
   std::vector<int64_t> gen;  // per slot: attach count, so a graph never outlives its problem
   int64_t gen_next = 0;
   std::vector<double> arg;        // per-slot scalar argument of the next launch
@@ -321,7 +323,7 @@ struct Engine {
     HIPC(hipHostMalloc((void **)&hscal, sizeof(double) * NSCAL * cap, hipHostMallocDefault));
     dlog = dalloc<double>((int64_t)cap * CG_LOG_ROUNDS * 5);
     HIPC(hipHostMalloc((void **)&hlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipHostMallocDefault));
-    graphs_on = getenv("PQDTS_GRAPHS") && atoi(getenv("PQDTS_GRAPHS")) != 0;
+    graphs_on = getenv("PQDTS_GRAPHS") ? std::max(0, std::min(2, atoi(getenv("PQDTS_GRAPHS")))) : 0;
   }
   void drop_graphs() {
     for (auto &kv : cg_graphs) (void)hipGraphExecDestroy(kv.second);
@@ -333,7 +335,15 @@ struct Engine {
   // launches are the eager ones, so the results are the same bits.
   template <class R>
   void graph_rounds(const std::vector<int> &S, int j0, int j1, R &&rounds) {
-    std::vector<int64_t> key{j0, j1, (int64_t)S.size()};
+    graph_run(S, j0, j1, [&] { rounds(j0, j1); });
+  }
+  // Runs the launches fn records for the problems S through a graph captured on
+  // first use for this key (tag0, tag1, the problems, their attach generations,
+  // spaces, precisions, Hessian paths and splits) and replayed afterwards. fn may
+  // change host state only in ways that the same key determines.
+  template <class Fn>
+  void graph_run(const std::vector<int> &S, int tag0, int tag1, Fn &&fn) {
+    std::vector<int64_t> key{tag0, tag1, (int64_t)S.size()};
     for (int p : S) {
       const pqdts_ctx *c = ctx[p];
       key.push_back(p);
@@ -347,7 +357,7 @@ struct Engine {
       hipGraph_t g = nullptr;
       HIPC(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        rounds(j0, j1);
+        fn();
       } catch (...) {
         (void)hipStreamEndCapture(s, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -1132,7 +1142,14 @@ void Engine::tick() {
     }
   }
   if (A.empty() && Q.empty()) return;
-  if (!A2.empty()) transform(A2, true);   // pqdts.f90:1970-1983
+  // PQDTS_GRAPHS=2: with every problem in the Newton-CG of mode 2 (none polishing),
+  // the launches before the line search and those after the step each replay a
+  // captured graph; the line search stays host-driven. The launches are the eager
+  // ones, except that all 50 CG rounds are launched (problems whose CG has stopped
+  // skip them at the guard) instead of leaving at a host check every 8 rounds.
+  const bool whole = graphs_on == 2 && !private_scal && !ktiming && !dev_timers && Q.empty() && A1.empty() &&
+                     !A2.empty();
+  if (!A2.empty() && !whole) transform(A2, true);   // pqdts.f90:1970-1983
   if (!A1.empty()) transform(A1, false);  // pqdts.f90:1940-1944
   // Polish (no counterpart in the reference): problems starting an outer iteration
   // (Qs; on entry the reduced-space transform of the minimisation's result),
@@ -1154,7 +1171,7 @@ void Engine::tick() {
     for (int p : Qent) ctx[p]->full = false;
     transform(Qent, true);
   }
-  std::vector<int> AQ = A;
+  std::vector<int> AQ = whole ? std::vector<int>() : A;
   AQ.insert(AQ.end(), Qs.begin(), Qs.end());
   if (!AQ.empty()) dL(AQ, X, G);
   std::vector<int> Qn, Fz;  // polish: new outer iteration, finished
@@ -1170,10 +1187,13 @@ void Engine::tick() {
       const double *h = host_scal(p);
       q.pol_nchg = (int)h[S_NCHG];
       q.pol_nact = (int64_t)h[S_NACT];
-      // converged: the active set repeats (the last solve was on this same face)
-      // and that solve's error, about its relative residual times its step, is
-      // within the target
-      if (q.pol_it > 0 && q.pol_nchg == 0 && q.pol_rtol * q.pol_step <= q.pol.tol) {
+      // converged: the active set repeats (the last solve was on this same face),
+      // that solve's step was a correction of at most 100 tol, and its own error,
+      // about its relative residual times that step, is within the target (the
+      // estimate alone can be off by the conditioning: a larger last step needs
+      // one more, loose, solve)
+      if (q.pol_it > 0 && q.pol_nchg == 0 && q.pol_step <= 100.0 * q.pol.tol &&
+          q.pol_rtol * q.pol_step <= q.pol.tol) {
         q.pol_status = 1;
         Fz.push_back(p);
       } else if (q.pol_it >= q.pol.max_outer || (q.pol_budget > 0 && q.pol_rounds >= q.pol_budget)) {
@@ -1199,9 +1219,34 @@ void Engine::tick() {
   std::vector<int> CG0 = A2;  // CG from a fresh start
   CG0.insert(CG0.end(), Qn.begin(), Qn.end());
   std::vector<int> Qstep;     // polish problems whose CG ended in this step
+  // round j of the PCG: r.z of the previous round is in S_B1 (j = 1), else S_RZ0 / S_RZ1
+  auto ib1_of = [](int j) { return j == 1 ? (int)S_B1 : ((j - 1) & 1) ? (int)S_RZ1 : (int)S_RZ0; };
+  if (whole) {
+    // transform, gradient, active set, PCG (pqdts.f90:1970-2090), gradient and the
+    // two-metric step (2093-2102) as one graph
+    Scope sc(this, "CG");
+    graph_run(A2, -1, 0, [&] {
+      transform(A2, true);
+      dL(A2, X, G);
+      cg_late = true;
+      cg_init(A2);
+      guard = S_CGACT;
+      for (int j = 1; j <= 50; j++) {
+        const int ib1 = ib1_of(j), i_rz = (j & 1) ? S_RZ1 : S_RZ0;
+        BF(A2, XT, HP, true, true, true, true);
+        cg_update(A2, ib1);
+        cg_dir(A2, ib1, i_rz, j);
+      }
+      guard = -1;
+      cg_late = false;
+      dL(A2, X, G);
+      twometric(A2);
+    });
+    for (int p : A2) ctx[p]->maxr_valid = !ctx[p]->full;  // what transform records on the host
+  }
   if (!CG0.empty() || !Qc.empty()) {
-    cg_late = true;
-    if (!CG0.empty()) cg_init(CG0);  // pqdts.f90:1986-2031
+    if (!whole) cg_late = true;
+    if (!CG0.empty() && !whole) cg_init(CG0);  // pqdts.f90:1986-2031
     if (!Qc.empty())
       launch(Qc, [](int) { return 1; }, [&](const WorkList &wl, int) {
         kb_cg_carry<<<wl.np, 64, 0, s>>>(dprobs, wl, S_CGN, S_RZ0, S_RZ1, S_B1);
@@ -1209,11 +1254,9 @@ void Engine::tick() {
     // PCG, pqdts.f90:2036-2090. The exit test runs on the device: every CG launch
     // skips problems whose flag dropped, and the host only checks every few rounds.
     // A polish CG runs at most 50 rounds per step too and continues in the next.
-    {
+    if (!whole) {
       Scope sc(this, "CG");
       guard = S_CGACT;
-      // round j: r.z of the previous round is in S_B1 (j = 1), else S_RZ0 / S_RZ1
-      auto ib1_of = [](int j) { return j == 1 ? (int)S_B1 : ((j - 1) & 1) ? (int)S_RZ1 : (int)S_RZ0; };
       // problems still in CG; those whose CG stopped leave the launches at the
       // next host check (their blocks would only exit at the guard)
       std::vector<int> C = CG0;
@@ -1227,7 +1270,7 @@ void Engine::tick() {
           cg_dir(C, ib1, i_rz, j);
         }
       };
-      const bool graph = graphs_on && !private_scal && !ktiming && !dev_timers;
+      const bool graph = graphs_on == 1 && !private_scal && !ktiming && !dev_timers;
       for (int j0 = 1; j0 <= 50; j0 += 8) {
         const int j1 = std::min(j0 + 7, 50);
         if (graph)
@@ -1274,7 +1317,7 @@ void Engine::tick() {
         Qstep.push_back(p);
       }
     }
-    if (!A2.empty()) {
+    if (!A2.empty() && !whole) {
       dL(A2, X, G);    // pqdts.f90:2093
       twometric(A2);   // pqdts.f90:2095-2102
     }
@@ -1300,6 +1343,17 @@ void Engine::tick() {
   // a polish stopped at a cap returns the minimisation's result unchanged
   std::vector<int> Fr, AF = A;
   for (int p : Fz) (st[p]->pol_status == 1 ? AF : Fr).push_back(p);
+  if (whole) {
+    graph_run(A, -2, 0, [&] {
+      project(A, X, false, false);
+      L(A, X, true);
+      dL(A, X, G);
+      pdot(A, G, G);
+      conv(A, X, G, false);
+    });
+    for (int p : A) ctx[p]->full = true;  // what conv records on the host
+    AF.clear();
+  }
   if (!AF.empty()) project(AF, X, false, false);
   if (!Fr.empty()) {
     axpby(Fr, X, PQDTS_Z, -1, 1.0, 0.0, AX_COPY, false);

@@ -113,15 +113,18 @@ def test_launch_forms_same_bits(precision):
     assert np.array_equal(xa, xb), f"max diff {np.max(np.abs(xa - xb)):.3e}"
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_graph_replay_same_bits(monkeypatch, precision):
-    """CG rounds replayed from captured hipGraphs (PQDTS_GRAPHS=1) give the bits of
-    the eager launches, over whole solves: graphs are replayed while the set of
-    problems in CG stays the same and captured anew when it changes."""
+def test_graph_replay_same_bits(monkeypatch, precision, mode):
+    """Launches replayed from captured hipGraphs give the bits and the events of the
+    eager launches, over whole solves. PQDTS_GRAPHS=1 captures the CG rounds in
+    blocks of 8; PQDTS_GRAPHS=2 captures a Newton iteration as two graphs around
+    the host-driven line search. Graphs are replayed while the set of problems and
+    their stages stay the same and captured anew when they change."""
     probs = problems()
 
     def run(graphs):
-        monkeypatch.setenv("PQDTS_GRAPHS", "1" if graphs else "0")
+        monkeypatch.setenv("PQDTS_GRAPHS", graphs)
         b = pqdts_amd.Batch(8)
         ctxs = []
         for p in probs:
@@ -130,13 +133,15 @@ def test_graph_replay_same_bits(monkeypatch, precision):
             ctxs.append(c)
             b.admit(c, 2, 300, p.eps, 1, 0, precision)
         res = b.run()
-        return [c.get("X") for c in ctxs], [res[i] for i in range(len(probs))]
+        ev = [list(b.events(i)) for i in range(len(probs))]
+        return [c.get("X") for c in ctxs], [res[i] for i in range(len(probs))], ev
 
-    xa, ra = run(False)
-    xb, rb = run(True)
+    xa, ra, ea = run("0")
+    xb, rb, eb = run(mode)
     for p, a, b_ in zip(probs, xa, xb):
         assert np.array_equal(a, b_), f"problem {p.name}"
     assert ra == rb
+    assert ea == eb
 
 
 def test_batch_membership_rules():
