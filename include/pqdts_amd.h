@@ -279,7 +279,10 @@ int pqdts_batch_kernel_stats(pqdts_batch *b, int cap, char (*names)[24], int *co
  * memory at creation; each lane's own engine arrays, about 2.4 KB per slot, are
  * not counted). lane_cus (may be NULL): per lane a consecutive CU range
  * lo, hi, or -1, -1 for all CUs. A request larger than the budget is refused at
- * submission. Every problem computes the same bits as solved alone. */
+ * submission. Every problem computes the same bits as solved alone. A lane's
+ * Newton iterations replay captured hipGraphs (the launches before the line
+ * search and after the step; PQDTS_GRAPHS in the environment overrides: 0 eager,
+ * 1 CG rounds in blocks of 8, 2 this). */
 typedef struct pqdts_server pqdts_server;
 int pqdts_server_create(int nlanes, const int64_t *lane_bounds, const int32_t *lane_cus, int capacity,
                         int64_t budget_bytes, pqdts_server **out);

@@ -227,6 +227,10 @@ int pqdts_server_create(int nlanes, const int64_t *lane_bounds, const int32_t *l
     for (int g = 0; g < nlanes; g++) {
       S->lanes.emplace_back(new Engine());
       S->lanes.back()->init(capacity, nullptr, true);
+      // a lane's Newton iterations replay captured graphs (PQDTS_GRAPHS=2) unless
+      // the environment chooses: the server times no kernels, and the graphs give
+      // the eager bits
+      if (!getenv("PQDTS_GRAPHS")) S->lanes.back()->graphs_on = 2;
       // restricted to CUs lo..hi (a consecutive range), before any worker starts
       if (lane_cus && lane_cus[2 * g] >= 0) {
         const int lo = lane_cus[2 * g], hi = lane_cus[2 * g + 1];
