@@ -210,6 +210,7 @@ HVP_SPLIT = -1  # --hvp-split: -1 leaves the library's choice
 POLISH = 0      # --polish: outer iterations of the polish after conv < eps (0: none)
 POLISH_CLASSES = 2  # --polish-classes
 POLISH_BUDGET = 1.0  # --polish-budget: polish CG rounds, at most this multiple of the solve's PCG rounds
+POLISH_METHOD = "direct"  # --polish-method: "direct" face solves or "cg" (Newton-CG steps)
 
 
 def make_ctx(pqdts_amd, prob, polish=True):
@@ -217,7 +218,7 @@ def make_ctx(pqdts_amd, prob, polish=True):
     if HVP_SPLIT >= 0:
         c.set_hvp_split(HVP_SPLIT)
     if POLISH > 0 and polish:
-        c.set_polish(POLISH, budget=POLISH_BUDGET, fp32=True)
+        c.set_polish(POLISH, budget=POLISH_BUDGET, fp32=True, method=POLISH_METHOD)
     return c
 
 
@@ -442,7 +443,8 @@ def certified(r):
 def polish_summary(res_m, res_b64, problems):
     if POLISH == 0:
         return None
-    out = {"max_outer": POLISH, "budget": POLISH_BUDGET, "size_classes_polished": POLISH_CLASSES,
+    out = {"method": POLISH_METHOD, "max_outer": POLISH, "budget": POLISH_BUDGET,
+           "size_classes_polished": POLISH_CLASSES,
            "not_polished": sum(1 for r in res_m if r["polish"]["status"] == 0),
            "stopped_uncertified": [{"M": problems[i].M, "N": problems[i].N, "gamma": problems[i].gamma,
                                     "mixed": res_m[i]["polish"], "fp64": res_b64[i]["polish"]}
@@ -825,7 +827,7 @@ def workload_served(args, pqdts_amd, workloads):
                 if POLISH > 0 and polished_req(nxt) != pol_now:
                     # the server applies its polish settings to the requests submitted after them
                     pol_now = polished_req(nxt)
-                    srv.set_polish(POLISH if pol_now else 0, budget=POLISH_BUDGET, fp32=True)
+                    srv.set_polish(POLISH if pol_now else 0, budget=POLISH_BUDGET, fp32=True, method=POLISH_METHOD)
                 ids[srv.submit(reqs[nxt], maxiter=args.maxiter, precision=precision, coo=coos[nxt])] = nxt
                 nxt += 1
             wait_ms = int(max(1.0, 1e3 * (arrivals[nxt] - now))) if nxt < n else 1000
@@ -850,7 +852,7 @@ def workload_served(args, pqdts_amd, workloads):
             (fr, fc, fv), (pr, pc, pv) = coos[i]
             c = pqdts_amd.Context(reqs[i].M, reqs[i].N, reqs[i].D, (fr, fc, fv), (pr, pc, pv), reqs[i].gamma)
             if POLISH > 0 and polished_req(i):
-                c.set_polish(POLISH, budget=POLISH_BUDGET)
+                c.set_polish(POLISH, budget=POLISH_BUDGET, method=POLISH_METHOD)
             c.pset(1.0 / reqs[i].N, "X")
             c.solve(2, args.maxiter, reqs[i].eps, 1, 0)
             X[i] = c.get("X")
@@ -882,7 +884,8 @@ def workload_served(args, pqdts_amd, workloads):
                             + ("were certified as the problem's minimiser by the polish (as was the FP64 "
                                "result), " if POLISH > 0 else "")
                             + "and are within ||dPi||inf <= 1e-6 of the plain FP64 result",
-        "polish": {"max_outer": POLISH, "budget": POLISH_BUDGET, "certified_mixed": int(sum(map(bool, cert_m))),
+        "polish": {"method": POLISH_METHOD, "max_outer": POLISH, "budget": POLISH_BUDGET,
+                   "certified_mixed": int(sum(map(bool, cert_m))),
                    "certified_fp64": int(sum(map(bool, cert_p)))} if POLISH > 0 else None,
         "converged_solves_per_s_ungated": sum(1 for c, p in zip(conv_m, reqs) if c < p.eps) / dt_m,
         "within_1e-6": gated,
@@ -949,12 +952,17 @@ def main():
                     help="polish the problems of this many of the three size classes (thirds by M x N), "
                          "smallest first; measured: the largest third is not certified within the budget")
     ap.add_argument("--polish-budget", type=float, default=1.0,
-                    help="polish CG rounds, at most this multiple of the minimisation's own PCG rounds")
+                    help="polish CG rounds, at most this multiple of the minimisation's own PCG rounds "
+                         "(the Newton-CG polish)")
+    ap.add_argument("--polish-method", choices=("direct", "cg"), default="direct",
+                    help="the polish's face solves: direct (column-block inverses and the row sums' band "
+                         "Schur complement) or Newton-CG steps")
     ap.add_argument("--hvp-split", type=int, default=-1,
                     help="split of the Gram product over band chunks (0 by shape, 1 none, k fixed)")
     args = ap.parse_args()
-    global HVP_SPLIT, POLISH, POLISH_BUDGET, POLISH_CLASSES
+    global HVP_SPLIT, POLISH, POLISH_BUDGET, POLISH_CLASSES, POLISH_METHOD
     HVP_SPLIT = args.hvp_split
+    POLISH_METHOD = args.polish_method
     # the polish applies to the mix (the metric's 1e-6 gate); the other workloads report
     # the reference's own stopping point
     POLISH, POLISH_BUDGET = (args.polish if args.workload in ("mix", "served") else 0), args.polish_budget

@@ -61,6 +61,7 @@ enum pqdts_event {
   PQDTS_EV_STAGE = 7,   /* oiter conv: end of a stage, before its output file  (pqdts.f90:2139) */
   /* polish (pqdts_set_polish; no counterpart in the reference) */
   PQDTS_EV_POLISH = 8,  /* iteration active_entries changed_entries cg_rounds step_max          */
+                        /* (method 1: then largest column block, band of S, factorisation failed) */
   PQDTS_EV_POLISHED = 9 /* status(1 converged, 2 cap) iterations cg_rounds conv L               */
 };
 typedef void (*pqdts_event_cb)(void *user, int kind, const double *values, int nvalues);
@@ -164,6 +165,23 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
  * Hessian products are FP64 unless fp32 = 1 and the solve is mixed precision.
  * max_outer = 0 turns it off (the default). */
 int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, double budget, int fp32);
+/* The polish's method (no counterpart in the reference): 0 (the default) as
+ * above; 1 direct face solves. Method 1 iterates in the full space with the row
+ * sums as constraints and multipliers lambda: an entry is held at 0 when
+ * G + lambda - d X > -1e-13 d (d the diagonal of K = 2 F^T F + 2 gamma Lap; a row keeps
+ * its largest entry free). On each new active set it forms the inverse of K on
+ * the free rows of every column and the Schur complement of the row sums (M x M,
+ * banded), factorises it (band Cholesky) and applies 2 corrections from the
+ * current point, each with a fresh gradient (iterative refinement), all in FP64;
+ * a column whose free rows did not change keeps its inverse.
+ * It stops when the active set repeats and the last correction is <= 100 tol
+ * (status 1: the unique minimiser), or with status 2 (X unchanged: the
+ * minimisation's result) after max_outer iterations, when M > 3072 or a column
+ * has more than 1024 free rows, or when the factorisation fails. cg_rtol, budget
+ * and fp32 do not apply. Its buffers, about 8 M N + 8 M^2 + 8 sum_n m_n^2 bytes
+ * (m_n the free rows of column n), are held while the polish runs and are not
+ * part of pqdts_footprint. */
+int pqdts_set_polish_method(pqdts_ctx *ctx, int method);
 /* outcome of the last polish on ctx: stats[0..2] = status (0 not run, 1 converged,
  * 2 stopped at a cap), iterations, CG rounds; *nact active entries, *step
  * the last step's largest entry (each pointer may be NULL) */
@@ -306,6 +324,8 @@ int pqdts_server_wait(pqdts_server *s, int64_t *ids, int cap, int timeout_ms);
 int pqdts_server_result(pqdts_server *s, int64_t id, double *X, double *conv, int *stats, double *times);
 /* polish settings (as pqdts_set_polish) for the requests submitted from now on */
 int pqdts_server_set_polish(pqdts_server *s, int max_outer, double cg_rtol, double tol, double budget, int fp32);
+/* polish method (as pqdts_set_polish_method) for the requests submitted from now on */
+int pqdts_server_set_polish_method(pqdts_server *s, int method);
 /* info[0..7] = queued, running, finished not yet waited for, device bytes held
  * now, peak device bytes held, budget, submitted, completed */
 int pqdts_server_info(pqdts_server *s, int64_t *info);

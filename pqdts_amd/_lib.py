@@ -32,10 +32,14 @@ EXPORTS = [
     "pqdts_device_cu_count", "pqdts_batch_set_cu_mask", "simplexproj_condat_",
     "pqdts_simplexproj_rows_noalloc", "pqdts_server_create", "pqdts_server_destroy", "pqdts_server_submit",
     "pqdts_server_wait", "pqdts_server_result", "pqdts_server_info", "pqdts_footprint", "pqdts_device_timers",
-    "pqdts_set_polish", "pqdts_polish_result", "pqdts_server_set_polish",
+    "pqdts_set_polish", "pqdts_polish_result", "pqdts_server_set_polish", "pqdts_set_polish_method",
+    "pqdts_server_set_polish_method",
 ]
 
 _lib = None
+
+# polish methods (pqdts_set_polish_method)
+POLISH_METHODS = {"cg": 0, "direct": 1}
 
 
 class PqdtsError(RuntimeError):
@@ -107,6 +111,8 @@ def load():
     L.pqdts_set_polish.argtypes = [vp, i32, d, d, d, i32]
     L.pqdts_polish_result.argtypes = [vp, C.POINTER(i32), i64p, dp]
     L.pqdts_server_set_polish.argtypes = [vp, i32, d, d, d, i32]
+    L.pqdts_set_polish_method.argtypes = [vp, i32]
+    L.pqdts_server_set_polish_method.argtypes = [vp, i32]
     L.pqdts_batch_create.argtypes = [i32, C.POINTER(vp)]
     L.pqdts_batch_destroy.argtypes = [vp]
     L.pqdts_batch_admit.argtypes = [vp, vp, i32, i32, d, i32, i32, i32, EVENT_CB, vp, C.POINTER(i32)]
@@ -344,12 +350,14 @@ class Context:
         """Split of the Gram product over band chunks: 0 by shape, 1 none, 2..16 fixed."""
         _check(load().pqdts_set_hvp_split(self.h, int(split)))
 
-    def set_polish(self, max_outer=100, cg_rtol=1e-6, tol=1e-10, budget=0.0, fp32=False):
+    def set_polish(self, max_outer=100, cg_rtol=1e-6, tol=1e-10, budget=0.0, fp32=False, method=0):
         """Polish every later solve of this context to the problem's minimiser (primal-dual
-        active-set Newton after conv < eps) to about `tol`, with at most `budget` times the
-        minimisation's PCG rounds (0: no limit); max_outer = 0 turns it off."""
+        active-set Newton after conv < eps) to about `tol`; max_outer = 0 turns it off.
+        method 0: Newton-CG steps, with at most `budget` times the minimisation's PCG rounds
+        (0: no limit); method 1 ("direct"): direct face solves."""
         _check(load().pqdts_set_polish(self.h, int(max_outer), float(cg_rtol), float(tol), float(budget),
                                        int(bool(fp32))))
+        _check(load().pqdts_set_polish_method(self.h, POLISH_METHODS.get(method, method)))
 
     def polish_result(self) -> dict:
         st = (C.c_int * 3)()
@@ -562,10 +570,11 @@ class Server:
                 "trials": stats[3], "queued_s": times[0], "run_s": times[1], "latency_s": times[2],
                 "polish": {"status": stats[4], "iterations": stats[5], "cg_rounds": stats[6]}}
 
-    def set_polish(self, max_outer=100, cg_rtol=1e-6, tol=1e-10, budget=0.0, fp32=False):
+    def set_polish(self, max_outer=100, cg_rtol=1e-6, tol=1e-10, budget=0.0, fp32=False, method=0):
         """Polish settings (Context.set_polish) for requests submitted from now on."""
         _check(load().pqdts_server_set_polish(self.h, int(max_outer), float(cg_rtol), float(tol), float(budget),
                                               int(bool(fp32))))
+        _check(load().pqdts_server_set_polish_method(self.h, POLISH_METHODS.get(method, method)))
 
     def info(self) -> dict:
         a = (C.c_int64 * 8)()

@@ -83,7 +83,7 @@ constexpr int CG_LOG_ROUNDS = 50;
 
 constexpr int NREG = 4;
 constexpr int LS_K = 2;                // line-search trials evaluated per host round trip, at most
-constexpr int NSCAL = 17 + 3 * LS_K;   // solver scalars, then per-trial objective terms and xmin
+constexpr int NSCAL = 21 + 3 * LS_K;   // solver scalars, then per-trial objective terms and xmin
 
 // flags of a problem within one launch
 constexpr unsigned F_FULL = 1u;  // full space (tfullspace); otherwise reduced

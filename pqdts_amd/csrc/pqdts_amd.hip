@@ -9,6 +9,7 @@
 // them at once. The plain solve is an engine with one problem.
 #include "../../include/pqdts_amd.h"
 #include "kernels.hip"
+#include "face.hip"
 
 #include <algorithm>
 #include <chrono>
@@ -64,17 +65,20 @@ constexpr int TILE = 64;  // rows and columns of a band-GEMM block tile
 // loosest relative CG residual of a polish iteration
 constexpr double kPolishLooseRtol = 1e-3;
 constexpr int KC = 16;    // band chunk staged per LDS buffer (4 MFMA k-steps)
+static_assert(KC == FACE_KC, "the direct polish reads G's band pack");
 
 // device scalar slots per problem
 enum {
   S_LA = 0, S_LB, S_XMIN, S_NACT, S_ND, S_B1, S_V2, S_RZ0, S_RZ1, S_RR, S_CONV, S_DOT,
   S_EPSCG, S_CGACT, S_CGN,  // device-side CG loop: eps, still-iterating flag, rounds done
   S_NCHG, S_STEP,            // polish: active entries that changed, minus the step's largest entry
-  S_LT = 17,                 // line-search trial k: data term S_LT + k, smoothing S_LTB + k, xmin S_LTX + k
+  S_FMAX, S_FTOT, S_FBW, S_FFAIL,  // direct polish: largest column block, total block size, band of
+                                   // the Schur complement, its factorisation failed
+  S_LT = 21,                 // line-search trial k: data term S_LT + k, smoothing S_LTB + k, xmin S_LTX + k
   S_LTB = S_LT + LS_K,
   S_LTX = S_LTB + LS_K
 };
-static_assert(S_STEP < S_LT && S_LTX + LS_K <= NSCAL, "scalar slots");
+static_assert(S_FFAIL < S_LT && S_LTX + LS_K <= NSCAL, "scalar slots");
 // partial-sum regions per problem
 enum { R_A = 0, R_B = 1, R_C = 2 };
 
@@ -169,6 +173,7 @@ struct pqdts_ctx {
     double budget = 0.0;      // CG rounds of the whole polish, at most this multiple of the
                               // minimisation's PCG rounds (<= 0: no limit)
     int fp32 = 0;             // 1: FP32 Hessian products where the solve is mixed
+    int method = 0;           // 0: Newton-CG in the reduced space; 1: direct face solves
   } pol;
   struct PolishOut {
     int status = 0;           // 0 not run, 1 converged, 2 stopped at a cap (result: the
@@ -262,6 +267,11 @@ struct Engine {
   double *dlog = nullptr;         // device-side CG logs [cap][CG_LOG_ROUNDS][5]
   double *hlog = nullptr;         // pinned host copy
   std::vector<int> fp32;          // per-slot: mixed precision Hessian products
+  // direct polish: per-slot buffers, their device copies and the capacity (doubles)
+  // of each slot's block storage
+  Face *dfaces = nullptr;
+  std::vector<Face> hface;
+  std::vector<int64_t> fkcap, fkcap_old;
   // timers: host wall clock around the (asynchronous) launches, or with
   // dev_timers the device time between HIP events recorded on the stream at the
   // start and the end of each section
@@ -320,6 +330,12 @@ struct Engine {
       s = stream;
     }
     dprobs = dalloc<Prob>(cap);
+    dfaces = dalloc<Face>(cap);
+    hface.assign(cap, Face{});
+    fkcap.assign(cap, 0);
+    fkcap_old.assign(cap, 0);
+    HIPC(hipMemcpyAsync(dfaces, hface.data(), sizeof(Face) * cap, hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
     HIPC(hipHostMalloc((void **)&hscal, sizeof(double) * NSCAL * cap, hipHostMallocDefault));
     dlog = dalloc<double>((int64_t)cap * CG_LOG_ROUNDS * 5);
     HIPC(hipHostMalloc((void **)&hlog, sizeof(double) * cap * CG_LOG_ROUNDS * 5, hipHostMallocDefault));
@@ -392,10 +408,73 @@ struct Engine {
     HIPC(hipStreamSynchronize(s));
   }
   void detach(int slot) {
+    face_free(slot);
     if (ctx[slot] && ctx[slot]->batch == this) ctx[slot]->batch = nullptr;
     ctx[slot] = nullptr;
     st[slot].reset();
   }
+
+  // ---- direct polish buffers (Face) of a slot
+  void face_upload_all() {
+    HIPC(hipMemcpyAsync(dfaces, hface.data(), sizeof(Face) * cap, hipMemcpyHostToDevice, s));
+    HIPC(hipStreamSynchronize(s));
+  }
+  void face_free(int p) {
+    if (p < 0 || p >= (int)hface.size()) return;
+    Face &f = hface[p];
+    if (!f.pos) return;
+    (void)hipStreamSynchronize(s);
+    void *bufs[] = {f.pos, f.crow, f.cnt, f.koff, f.kb, f.koff_old, f.kb_old, f.cdirty, f.cfresh,
+                    f.S, f.lam, f.rhs, f.slast, f.cstep};
+    for (void *q : bufs)
+      if (q) (void)hipFree(q);
+    f = Face{};
+    fkcap[p] = fkcap_old[p] = 0;
+  }
+  // every buffer but the block storage; every column's block is formed in the first
+  // iteration (cdirty = 1)
+  void face_alloc(int p) {
+    face_free(p);
+    const int64_t M = ctx[p]->M, N = ctx[p]->N;
+    Face &f = hface[p];
+    try {
+      f.pos = dalloc<int>(M * N);
+      f.crow = dalloc<int>(M * N);
+      f.cnt = dalloc<int>(N);
+      f.koff = dalloc<int64_t>(N + 1);
+      f.koff_old = dalloc<int64_t>(N + 1);
+      f.cdirty = dalloc<uint8_t>(N);
+      f.cfresh = dalloc<uint8_t>(N);
+      f.S = dalloc<double>(M * M);
+      f.lam = dalloc<double>(M);
+      f.rhs = dalloc<double>(M);
+      f.slast = dalloc<int>(M);
+      f.cstep = dalloc<double>(N);
+      HIPC(hipMemsetAsync(f.cdirty, 1, (size_t)N, s));
+    } catch (...) {
+      face_free(p);
+      throw;
+    }
+  }
+  // the last iteration's blocks become the old ones
+  void face_swap(int p) {
+    Face &f = hface[p];
+    std::swap(f.kb, f.kb_old);
+    std::swap(f.koff, f.koff_old);
+    std::swap(fkcap[p], fkcap_old[p]);
+  }
+  // block storage for `need` doubles (the stream is idle: called after a fetch)
+  void face_grow(int p, int64_t need) {
+    if (need <= fkcap[p]) return;
+    Face &f = hface[p];
+    if (f.kb) HIPC(hipFree(f.kb));
+    f.kb = nullptr;
+    fkcap[p] = 0;
+    const int64_t c2 = need + need / 4 + 1024;
+    f.kb = dalloc<double>(c2);
+    fkcap[p] = c2;
+  }
+  void face_step(const std::vector<int> &QD, std::vector<int> &Fz);
 
   // ---- timing (never throws: tstop runs in destructors)
   hipEvent_t tevent() {
@@ -967,6 +1046,8 @@ Engine::~Engine() {
     (void)hipEventDestroy(p.b);
   }
   for (hipEvent_t e : kpool) (void)hipEventDestroy(e);
+  for (int p = 0; p < (int)hface.size(); p++) face_free(p);
+  if (dfaces) (void)hipFree(dfaces);
   if (dprobs) (void)hipFree(dprobs);
   if (dscal) (void)hipFree(dscal);
   if (hscal) (void)hipHostFree(hscal);
@@ -1059,6 +1140,134 @@ void Engine::line_search(const std::vector<int> &A) {
 
 // One Newton iteration (or the initial evaluation / stage switches) for every
 // problem of the engine, pqdts.f90:1908-2167.
+// corrections (face solves from the current point) per iteration of the direct polish
+constexpr int kFacePasses = 2;
+
+// Direct polish (polish method 1; no counterpart in the reference): one
+// primal-dual active-set iteration per step for the problems QD, in the full space
+// with the row sums as constraints. On entry the minimisation's result is kept in
+// Z and the multipliers start at -G at each row's largest entry. Each iteration
+// takes the active set from mu - d X > -theta d (mu = G + lambda) and, unless it repeats
+// the face just solved, forms the inverses of K on every column's free rows, the
+// Schur complement of the row sums and its band Cholesky factor, then applies
+// kFacePasses corrections from the current point, each with a fresh gradient.
+// Finished problems go to Fz with their status: 1 when the active set repeats
+// after a last correction of at most 100 tol, 2 at the iteration cap, for a
+// problem larger than the method handles (M > FACE_MAX_ROWS or a column with more
+// than FACE_MAX_M free rows), or when the factorisation fails.
+void Engine::face_step(const std::vector<int> &QD, std::vector<int> &Fz) {
+  Scope sc(this, "polish");
+  const int X = PQDTS_X, G = PQDTS_G, Y = PQDTS_HP;
+  // launches timed like the operators' (HIP events, when kernel timing is on)
+  auto timed = [&](const char *name, const std::vector<int> &S, auto nb, auto fn) {
+    hipEvent_t e0 = kbegin();
+    launch(S, nb, fn);
+    kend(e0, name, S, [](int) { return Work{}; });
+  };
+  std::vector<int> ent, init, Qa;
+  for (int p : QD)
+    if (!st[p]->pol_entered) {
+      st[p]->pol_entered = true;
+      ent.push_back(p);
+    }
+  // the minimisation's result (full space): the result if the polish stops at a cap
+  if (!ent.empty()) axpby(ent, PQDTS_Z, X, -1, 1.0, 0.0, AX_COPY, false);
+  for (int p : ent) {
+    if (ctx[p]->M > FACE_MAX_ROWS || !ctx[p]->full) {
+      st[p]->pol_status = 2;
+      Fz.push_back(p);
+    } else {
+      face_alloc(p);
+      init.push_back(p);
+    }
+  }
+  if (!init.empty()) face_upload_all();
+  if (!init.empty()) {
+    dL(init, X, G);
+    timed("kf_lam0", init, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kf_lam0<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl);
+    });
+  }
+  for (int p : QD)
+    if (st[p]->pol_status == 0) Qa.push_back(p);
+  if (Qa.empty()) return;
+  for (int p : Qa) face_swap(p);
+  face_upload_all();
+  timed("kf_active", Qa, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+    kf_active<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl);
+    kb_reduce_pn<<<wl.np, BLOCK, 0, s>>>(dprobs, wl, 2, R_A, S_NCHG, R_B, S_NACT, 0, 0, 0);
+  });
+  timed("kf_cols", Qa, [&](int p) { return cdiv(ctx[p]->N, BLOCK); }, [&](const WorkList &wl, int tot) {
+    kf_cols<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl);
+  });
+  timed("kf_offsets", Qa, [](int) { return 1; }, [&](const WorkList &wl, int tot) {
+    kf_offsets<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl, S_FMAX, S_FTOT);
+  });
+  fetch();
+  std::vector<int> go;
+  for (int p : Qa) {
+    PState &q = *st[p];
+    const double *h = host_scal(p);
+    q.pol_nchg = (int)h[S_NCHG];
+    q.pol_nact = (int64_t)h[S_NACT];
+    if (q.pol_it > 0 && q.pol_nchg == 0 && q.pol_step <= 100.0 * q.pol.tol) {
+      q.pol_status = 1;
+      Fz.push_back(p);
+    } else if (q.pol_it >= q.pol.max_outer || h[S_FMAX] > FACE_MAX_M) {
+      q.pol_status = 2;
+      Fz.push_back(p);
+    } else {
+      q.pol_it++;
+      face_grow(p, (int64_t)h[S_FTOT]);
+      go.push_back(p);
+    }
+  }
+  if (go.empty()) return;
+  face_upload_all();
+  timed("kf_factor", go, [&](int p) { return ctx[p]->N; }, [&](const WorkList &wl, int tot) {
+    kf_factor<FACE_LDS_S><<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl);
+    kf_factor<FACE_LDS_L><<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl);
+    kf_factor_big<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl);
+  });
+  timed("kf_schur", go, [&](int p) { return ctx[p]->M; }, [&](const WorkList &wl, int tot) {
+    kf_schur<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl);
+  });
+  timed("kf_chol", go, [](int) { return 1; }, [&](const WorkList &wl, int tot) {
+    kf_chol<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl, S_FBW, S_FFAIL);
+  });
+  for (int pass = 0; pass < kFacePasses; pass++) {
+    dL(go, X, G);
+    timed("kf_y", go, [&](int p) { return ctx[p]->N; }, [&](const WorkList &wl, int tot) {
+      kf_y<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl, Y);
+    });
+    timed("kf_rhs", go, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kf_rhs<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl, Y);
+    });
+    timed("kf_solve", go, [](int) { return 1; }, [&](const WorkList &wl, int tot) {
+      kf_solve<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl, S_FBW);
+    });
+    timed("kf_update", go, [&](int p) { return ctx[p]->N; }, [&](const WorkList &wl, int tot) {
+      kf_update<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl, Y);
+    });
+    timed("kf_stepmax", go, [](int) { return 1; }, [&](const WorkList &wl, int tot) {
+      kf_stepmax<<<tot, BLOCK, 0, s>>>(dprobs, dfaces, wl, S_STEP);
+    });
+  }
+  dL(go, X, G);  // the gradient at the new point, for the next active set
+  fetch();
+  for (int p : go) {
+    PState &q = *st[p];
+    const double *h = host_scal(p);
+    q.pol_step = -h[S_STEP];
+    q.emit(PQDTS_EV_POLISH, {(double)q.pol_it, (double)q.pol_nact, (double)q.pol_nchg, 0.0, q.pol_step,
+                             h[S_FMAX], h[S_FBW], h[S_FFAIL]});
+    if (h[S_FFAIL] != 0.0 || !std::isfinite(q.pol_step)) {
+      q.pol_status = 2;
+      Fz.push_back(p);
+    }
+  }
+}
+
 void Engine::tick() {
   const int X = PQDTS_X, G = PQDTS_G, HP = PQDTS_HP, XT = PQDTS_XTMP, SN = PQDTS_SN;
   std::vector<int> init;
@@ -1151,6 +1360,14 @@ void Engine::tick() {
                      !A2.empty();
   if (!A2.empty() && !whole) transform(A2, true);   // pqdts.f90:1970-1983
   if (!A1.empty()) transform(A1, false);  // pqdts.f90:1940-1944
+  std::vector<int> Fz;  // polish: finished in this step
+  {
+    // the direct polish runs one active-set iteration per step on its own
+    std::vector<int> QD, QK;
+    for (int p : Q) (st[p]->pol.method == 1 ? QD : QK).push_back(p);
+    if (!QD.empty()) face_step(QD, Fz);
+    Q.swap(QK);
+  }
   // Polish (no counterpart in the reference): problems starting an outer iteration
   // (Qs; on entry the reduced-space transform of the minimisation's result),
   // problems whose CG continues from the last step (Qc)
@@ -1174,7 +1391,7 @@ void Engine::tick() {
   std::vector<int> AQ = whole ? std::vector<int>() : A;
   AQ.insert(AQ.end(), Qs.begin(), Qs.end());
   if (!AQ.empty()) dL(AQ, X, G);
-  std::vector<int> Qn, Fz;  // polish: new outer iteration, finished
+  std::vector<int> Qn;  // polish: new outer iteration
   if (!Qs.empty()) {
     Scope sc(this, "polish");
     launch(Qs, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
@@ -1341,8 +1558,11 @@ void Engine::tick() {
     }
   if (!stepS.empty()) axpby(stepS, X, SN, X, 0.0, 0.0, AX_AXPY, true);
   // a polish stopped at a cap returns the minimisation's result unchanged
-  std::vector<int> Fr, AF = A;
-  for (int p : Fz) (st[p]->pol_status == 1 ? AF : Fr).push_back(p);
+  // AD: certified direct polishes, whose result is feasible to rounding: clipped at
+  // 0 instead of projected (a projection would spread the rows' rounding-level sum
+  // error over every entry, zeros included)
+  std::vector<int> Fr, AF = A, AD;
+  for (int p : Fz) (st[p]->pol_status == 1 ? (st[p]->pol.method == 1 ? AD : AF) : Fr).push_back(p);
   if (whole) {
     graph_run(A, -2, 0, [&] {
       project(A, X, false, false);
@@ -1355,6 +1575,12 @@ void Engine::tick() {
     AF.clear();
   }
   if (!AF.empty()) project(AF, X, false, false);
+  if (!AD.empty()) {
+    launch(AD, [&](int p) { return nb_rows(p); }, [&](const WorkList &wl, int tot) {
+      kf_clip<<<tot, BLOCK, 0, s>>>(dprobs, wl);
+    });
+    AF.insert(AF.end(), AD.begin(), AD.end());
+  }
   if (!Fr.empty()) {
     axpby(Fr, X, PQDTS_Z, -1, 1.0, 0.0, AX_COPY, false);
     for (int p : Fr) ctx[p]->full = true;
@@ -1386,6 +1612,7 @@ void Engine::tick() {
     PState &q = *st[p];
     q.conv = convval(p);
     q.phase = 2;
+    face_free(p);
     pqdts_ctx::PolishOut &o = ctx[p]->pol_out;
     o.status = q.pol_status;
     o.iters = q.pol_it;
@@ -2308,6 +2535,15 @@ int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, 
     ctx->pol.tol = tol;
     ctx->pol.budget = budget;
     ctx->pol.fp32 = fp32;
+  });
+}
+
+int pqdts_set_polish_method(pqdts_ctx *ctx, int method) {
+  return guarded([&] {
+    checked(ctx);
+    require(method == 0 || method == 1, "bad polish method (0 Newton-CG, 1 direct face solves)");
+    require(ctx->batch == nullptr, "the context is in a batch (release it first)");
+    ctx->pol.method = method;
   });
 }
 

@@ -372,6 +372,15 @@ int pqdts_server_set_polish(pqdts_server *s, int max_outer, double cg_rtol, doub
   });
 }
 
+int pqdts_server_set_polish_method(pqdts_server *s, int method) {
+  return guarded([&] {
+    require(s != nullptr, "null server");
+    require(method == 0 || method == 1, "bad polish method (0 Newton-CG, 1 direct face solves)");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->pol.method = method;
+  });
+}
+
 int pqdts_server_info(pqdts_server *s, int64_t *info) {
   return guarded([&] {
     require(s != nullptr && info != nullptr, "null argument");
