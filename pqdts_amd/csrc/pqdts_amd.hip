@@ -272,6 +272,27 @@ struct Engine {
   Face *dfaces = nullptr;
   std::vector<Face> hface;
   std::vector<int64_t> fkcap, fkcap_old;
+  // Stream-ordered allocation (hipMallocAsync) for the polish's buffers where the
+  // device supports memory pools: hipFree waits for the whole device, which would
+  // stall the other lanes' streams every time a problem enters or leaves the polish.
+  bool pools = false;
+  template <class T>
+  T *salloc(int64_t n) {
+    void *p = nullptr;
+    const size_t b = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
+    if (pools)
+      HIPC(hipMallocAsync(&p, b, s));
+    else
+      HIPC(hipMalloc(&p, b));
+    return (T *)p;
+  }
+  void sfree(void *p) {
+    if (!p) return;
+    if (pools)
+      (void)hipFreeAsync(p, s);
+    else
+      (void)hipFree(p);
+  }
   // timers: host wall clock around the (asynchronous) launches, or with
   // dev_timers the device time between HIP events recorded on the stream at the
   // start and the end of each section
@@ -331,6 +352,19 @@ struct Engine {
     }
     dprobs = dalloc<Prob>(cap);
     dfaces = dalloc<Face>(cap);
+    {
+      int dev = 0, v = 0;
+      HIPC(hipGetDevice(&dev));
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMemoryPoolsSupported, dev) == hipSuccess && v) {
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+          // keep freed blocks in the pool for the next problems' polish
+          uint64_t keep = UINT64_MAX;
+          (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+          pools = true;
+        }
+      }
+    }
     hface.assign(cap, Face{});
     fkcap.assign(cap, 0);
     fkcap_old.assign(cap, 0);
@@ -423,11 +457,9 @@ struct Engine {
     if (p < 0 || p >= (int)hface.size()) return;
     Face &f = hface[p];
     if (!f.pos) return;
-    (void)hipStreamSynchronize(s);
     void *bufs[] = {f.pos, f.crow, f.cnt, f.koff, f.kb, f.koff_old, f.kb_old, f.cdirty, f.cfresh,
                     f.S, f.lam, f.rhs, f.slast, f.cstep};
-    for (void *q : bufs)
-      if (q) (void)hipFree(q);
+    for (void *q : bufs) sfree(q);
     f = Face{};
     fkcap[p] = fkcap_old[p] = 0;
   }
@@ -438,18 +470,18 @@ struct Engine {
     const int64_t M = ctx[p]->M, N = ctx[p]->N;
     Face &f = hface[p];
     try {
-      f.pos = dalloc<int>(M * N);
-      f.crow = dalloc<int>(M * N);
-      f.cnt = dalloc<int>(N);
-      f.koff = dalloc<int64_t>(N + 1);
-      f.koff_old = dalloc<int64_t>(N + 1);
-      f.cdirty = dalloc<uint8_t>(N);
-      f.cfresh = dalloc<uint8_t>(N);
-      f.S = dalloc<double>(M * M);
-      f.lam = dalloc<double>(M);
-      f.rhs = dalloc<double>(M);
-      f.slast = dalloc<int>(M);
-      f.cstep = dalloc<double>(N);
+      f.pos = salloc<int>(M * N);
+      f.crow = salloc<int>(M * N);
+      f.cnt = salloc<int>(N);
+      f.koff = salloc<int64_t>(N + 1);
+      f.koff_old = salloc<int64_t>(N + 1);
+      f.cdirty = salloc<uint8_t>(N);
+      f.cfresh = salloc<uint8_t>(N);
+      f.S = salloc<double>(M * M);
+      f.lam = salloc<double>(M);
+      f.rhs = salloc<double>(M);
+      f.slast = salloc<int>(M);
+      f.cstep = salloc<double>(N);
       HIPC(hipMemsetAsync(f.cdirty, 1, (size_t)N, s));
     } catch (...) {
       face_free(p);
@@ -463,15 +495,15 @@ struct Engine {
     std::swap(f.koff, f.koff_old);
     std::swap(fkcap[p], fkcap_old[p]);
   }
-  // block storage for `need` doubles (the stream is idle: called after a fetch)
+  // block storage for `need` doubles
   void face_grow(int p, int64_t need) {
     if (need <= fkcap[p]) return;
     Face &f = hface[p];
-    if (f.kb) HIPC(hipFree(f.kb));
+    sfree(f.kb);
     f.kb = nullptr;
     fkcap[p] = 0;
     const int64_t c2 = need + need / 4 + 1024;
-    f.kb = dalloc<double>(c2);
+    f.kb = salloc<double>(c2);
     fkcap[p] = c2;
   }
   void face_step(const std::vector<int> &QD, std::vector<int> &Fz);
@@ -1047,6 +1079,7 @@ Engine::~Engine() {
   }
   for (hipEvent_t e : kpool) (void)hipEventDestroy(e);
   for (int p = 0; p < (int)hface.size(); p++) face_free(p);
+  if (s) (void)hipStreamSynchronize(s);
   if (dfaces) (void)hipFree(dfaces);
   if (dprobs) (void)hipFree(dprobs);
   if (dscal) (void)hipFree(dscal);
