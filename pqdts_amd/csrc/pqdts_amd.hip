@@ -270,7 +270,7 @@ struct Engine {
   // direct polish: per-slot buffers, their device copies and the capacity (doubles)
   // of each slot's block storage
   Face *dfaces = nullptr;
-  std::vector<Face> hface;
+  Face *hface = nullptr;  // pinned host copy of dfaces (cap entries)
   std::vector<int64_t> fkcap, fkcap_old;
   // Stream-ordered allocation (hipMallocAsync) for the polish's buffers where the
   // device supports memory pools: hipFree waits for the whole device, which would
@@ -365,10 +365,11 @@ struct Engine {
         }
       }
     }
-    hface.assign(cap, Face{});
+    HIPC(hipHostMalloc((void **)&hface, sizeof(Face) * cap, hipHostMallocDefault));
+    for (int p = 0; p < cap; p++) hface[p] = Face{};
     fkcap.assign(cap, 0);
     fkcap_old.assign(cap, 0);
-    HIPC(hipMemcpyAsync(dfaces, hface.data(), sizeof(Face) * cap, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dfaces, hface, sizeof(Face) * cap, hipMemcpyHostToDevice, s));
     HIPC(hipStreamSynchronize(s));
     HIPC(hipHostMalloc((void **)&hscal, sizeof(double) * NSCAL * cap, hipHostMallocDefault));
     dlog = dalloc<double>((int64_t)cap * CG_LOG_ROUNDS * 5);
@@ -450,11 +451,11 @@ struct Engine {
 
   // ---- direct polish buffers (Face) of a slot
   void face_upload_all() {
-    HIPC(hipMemcpyAsync(dfaces, hface.data(), sizeof(Face) * cap, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dfaces, hface, sizeof(Face) * cap, hipMemcpyHostToDevice, s));
     HIPC(hipStreamSynchronize(s));
   }
   void face_free(int p) {
-    if (p < 0 || p >= (int)hface.size()) return;
+    if (!hface || p < 0 || p >= cap) return;
     Face &f = hface[p];
     if (!f.pos) return;
     void *bufs[] = {f.pos, f.crow, f.cnt, f.koff, f.kb, f.koff_old, f.kb_old, f.cdirty, f.cfresh,
@@ -1078,8 +1079,9 @@ Engine::~Engine() {
     (void)hipEventDestroy(p.b);
   }
   for (hipEvent_t e : kpool) (void)hipEventDestroy(e);
-  for (int p = 0; p < (int)hface.size(); p++) face_free(p);
+  for (int p = 0; p < cap; p++) face_free(p);
   if (s) (void)hipStreamSynchronize(s);
+  if (hface) (void)hipHostFree(hface);
   if (dfaces) (void)hipFree(dfaces);
   if (dprobs) (void)hipFree(dprobs);
   if (dscal) (void)hipFree(dscal);
