@@ -398,3 +398,40 @@ def test_qp_certificate_rejects_a_perturbed_minimiser():
     Y[7, zero[0]] += 1e-6
     Y[7, free[0]] -= 1e-6
     assert not qp.is_minimiser(p.F, p.P, p.gamma, Y)
+
+
+def test_qp_face_polish_matches_brute_force_enumeration():
+    """The direct polish's iteration (full-space active-set steps, column-block
+    inverses, the row sums' Schur complement, refinement on the multiplier increment),
+    restated densely in oracle/qp.py, ends at the enumerated minimiser of 40 random
+    tiny problems (gamma 0 with F of full column rank, 1e-3 and 0.1), certified."""
+    from oracle import qp
+    rng = np.random.default_rng(5)
+    done = 0
+    while done < 40:
+        M, N = int(rng.integers(2, 5)), int(rng.integers(2, 4))
+        if M * N > 12:
+            continue
+        D = M + int(rng.integers(1, 4))
+        F = sps.csr_matrix(rng.random((D, M)) + 0.05)
+        P = rng.random((D, N))
+        gamma = float(rng.choice([0.0, 1e-3, 0.1]))
+        Xb = qp.brute_force(F, P, gamma)
+        X, status = qp.face_polish(F, P, gamma, np.full((M, N), 1.0 / N))
+        assert status == 1, (M, N, D, gamma)
+        assert np.max(np.abs(X - Xb)) <= 1e-9, (M, N, D, gamma)
+        done += 1
+
+
+def test_qp_face_polish_matches_the_exact_solution_on_mix_problems():
+    """On mix problems 162, 245 and 214 (M = 78-88, gamma 5e-4-8e-3), from the uniform
+    start: certified, within 1e-12 of qp.exact, row sums exact to 1e-15 and
+    stationarity below 1e-12."""
+    from oracle import qp
+    for p in workloads.batch_mix(256, eps=1e-7, only=[162, 245, 214], workers=1):
+        X, status = qp.face_polish(p.F, p.P, p.gamma, np.full((p.M, p.N), 1.0 / p.N))
+        assert status == 1, p.name
+        Xe, ok = qp.exact(p.F, p.P, p.gamma, X0=X)
+        assert ok and np.max(np.abs(X - Xe)) <= 1e-12, p.name
+        k = qp.kkt(p.F, p.P, p.gamma, X)
+        assert k["x_min"] >= 0.0 and k["rowsum_err"] <= 1e-15 and k["stationarity"] <= 1e-12, (p.name, k)
