@@ -254,8 +254,8 @@ __global__ __launch_bounds__(BLOCK) void kf_offsets(const Prob *__restrict__ pro
 constexpr int FACE_LDS_S = 48;
 constexpr int FACE_LDS_L = 140;
 
-// One workgroup per column, for the blocks of one size class (LM = FACE_LDS_S or
-// FACE_LDS_L: in LDS; 0: global memory, above FACE_LDS_L). A column whose free
+// One workgroup per column, for the blocks of one size class held in LDS (LM =
+// FACE_LDS_S or FACE_LDS_L; kf_factor_big takes the larger ones). A column whose free
 // set did not change since the last factorisation (cfresh[n] = 0) copies its
 // inverse from the previous block storage (kb_old at koff_old[n]). Otherwise: the
 // block K[I_n, I_n], then its inverse in place by symmetric sweeps (Gauss-Jordan
