@@ -51,3 +51,32 @@ def test_certified_reads_the_polish_status():
     finally:
         bench.POLISH = 0
     assert bench.certified({})
+
+
+def test_make_ctx_passes_the_polish_method():
+    """The bench's contexts get the polish method of --polish-method (the direct face
+    solves by default) and only where the polish applies."""
+    calls = []
+
+    class Ctx:
+        @classmethod
+        def from_problem(cls, prob):
+            return cls()
+
+        def set_polish(self, *a, **k):
+            calls.append((a, k))
+
+    fake = types.SimpleNamespace(Context=Ctx)
+    saved = bench.POLISH, bench.POLISH_METHOD
+    try:
+        bench.POLISH, bench.POLISH_METHOD = 60, "direct"
+        bench.make_ctx(fake, None, polish=True)
+        bench.make_ctx(fake, None, polish=False)
+        bench.POLISH_METHOD = "cg"
+        bench.make_ctx(fake, None, polish=True)
+    finally:
+        bench.POLISH, bench.POLISH_METHOD = saved
+    assert [k["method"] for _, k in calls] == ["direct", "cg"]
+    assert all(a == (60,) for a, _ in calls)
+    from pqdts_amd import _lib
+    assert _lib.POLISH_METHODS == {"cg": 0, "direct": 1}
