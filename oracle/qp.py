@@ -175,7 +175,7 @@ def brute_force(F, P, gamma: float):
 
 
 def face_polish(F, P, gamma: float, X0, tol: float = 1e-10, max_outer: int = 60, passes: int = 2,
-                theta: float = 1e-13):
+                theta: float = 1e-13, globalise: bool = False, memory: int = 5, with_iterations: bool = False):
     """The direct polish's iteration (polish method 1 of the library), restated densely:
     primal-dual active-set steps in the full space with the row sums as constraints.
     The active set of a step is {G + lambda - d X > -theta d} (d = diag K), a row whose
@@ -183,7 +183,14 @@ def face_polish(F, P, gamma: float, X0, tol: float = 1e-10, max_outer: int = 60,
     each row's largest entry. Each face is solved by `passes` corrections from the
     current point with the column-block inverses and the Schur complement of the row
     sums, solving for the multiplier increment. Returns (X, status): 1 when the active
-    set repeats after a last correction of at most 100 tol, 2 at the iteration cap."""
+    set repeats after a last correction of at most 100 tol, 2 at the iteration cap
+    (and the iteration count too with `with_iterations`).
+
+    globalise: the face solution is taken as a direction instead of the next point, and
+    the step is halved until the complementarity residual ||min(mu, d X)/d||^2 +
+    ||1 - X 1||^2 falls below the largest of its last `memory` values (a nonmonotone
+    Armijo test, sigma = 1e-4); certified when the active set repeats with that residual
+    below 1e-24. Not in the library."""
     K = hessian_block(F, gamma).toarray()
     B = 2.0 * (sps.csr_matrix(F).T @ np.asarray(P))
     X = np.array(X0, dtype=np.float64)
@@ -192,33 +199,58 @@ def face_polish(F, P, gamma: float, X0, tol: float = 1e-10, max_outer: int = 60,
     d = np.diag(K)[:, None]
     g = K @ X - B
     lam = -g[rows, X.argmax(1)]
-    act, step = None, np.inf
+
+    def resid(X, g, lam):
+        r1 = np.minimum(g + lam[:, None], d * X) / d
+        r2 = 1.0 - X.sum(1)
+        return float((r1 ** 2).sum() + (r2 ** 2).sum())
+
+    def out(X, status, it):
+        return (X, status, it) if with_iterations else (X, status)
+
+    act, step, hist = None, np.inf, []
     for it in range(max_outer + 1):
         new = (g + lam[:, None]) - d * X > -theta * d
         full_rows = new.all(1)
         new[rows[full_rows], X.argmax(1)[full_rows]] = False
-        if act is not None and np.array_equal(new, act) and step <= 100.0 * tol:
-            return X, 1
+        r0 = resid(X, g, lam) if globalise else 0.0
+        done = step <= 100.0 * tol if not globalise else r0 < 1e-24
+        if act is not None and np.array_equal(new, act) and done:
+            return out(X, 1, it)
         if it == max_outer:
-            return X, 2
+            return out(X, 2, it)
         act = new
-        X[act] = 0.0
+        Xf = X.copy()
+        Xf[act] = 0.0
+        lf = lam.copy()
         inv, S = _face_factors(K, ~act)
         L = np.linalg.cholesky(S)
         for _ in range(passes):
-            g = K @ X - B
+            gf = K @ Xf - B
             Y = np.zeros((M, N))
-            gl = g + lam[:, None]
+            gl = gf + lf[:, None]
             for n, (I, Ki) in enumerate(inv):
                 if I.size:
                     Y[I, n] = Ki @ gl[I, n]
-            dl = np.linalg.solve(L.T, np.linalg.solve(L, -(1.0 - X.sum(1)) - Y.sum(1)))
-            lam = lam + dl
+            dl = np.linalg.solve(L.T, np.linalg.solve(L, -(1.0 - Xf.sum(1)) - Y.sum(1)))
+            lf = lf + dl
             dX = np.zeros((M, N))
             for n, (I, Ki) in enumerate(inv):
                 if I.size:
                     dX[I, n] = -(Y[I, n] + Ki @ dl[I])
-            X += dX
+            Xf += dX
             step = float(np.abs(dX).max())
-        g = K @ X - B
-    return X, 2
+        gf = K @ Xf - B
+        if not globalise:
+            X, g, lam = Xf, gf, lf
+            continue
+        hist.append(r0)
+        ref = max(hist[-memory:])
+        t = 1.0
+        while True:
+            Xt, gt, lt = X + t * (Xf - X), g + t * (gf - g), lam + t * (lf - lam)
+            if resid(Xt, gt, lt) <= (1.0 - 2e-4 * t) * ref or t < 1e-3:
+                break
+            t *= 0.5
+        X, g, lam = Xt, gt, lt
+    return out(X, 2, max_outer)

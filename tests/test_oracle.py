@@ -435,3 +435,17 @@ def test_qp_face_polish_matches_the_exact_solution_on_mix_problems():
         assert ok and np.max(np.abs(X - Xe)) <= 1e-12, p.name
         k = qp.kkt(p.F, p.P, p.gamma, X)
         assert k["x_min"] >= 0.0 and k["rowsum_err"] <= 1e-15 and k["stationarity"] <= 1e-12, (p.name, k)
+
+
+def test_qp_face_polish_globalised_matches_the_exact_solution():
+    """The globalised variant of the restated iteration (the face solution as a
+    direction, halved under a nonmonotone Armijo test on the complementarity residual)
+    also ends at qp.exact's minimiser, certified, on mix problems 162, 245 and 214."""
+    from oracle import qp
+    for p in workloads.batch_mix(256, eps=1e-7, only=[162, 245, 214], workers=1):
+        X, status = qp.face_polish(p.F, p.P, p.gamma, np.full((p.M, p.N), 1.0 / p.N), globalise=True)
+        assert status == 1, p.name
+        Xe, ok = qp.exact(p.F, p.P, p.gamma, X0=X)
+        assert ok and np.max(np.abs(X - Xe)) <= 1e-12, p.name
+        k = qp.kkt(p.F, p.P, p.gamma, X)
+        assert k["x_min"] >= 0.0 and k["rowsum_err"] <= 1e-15 and k["stationarity"] <= 1e-12, (p.name, k)
