@@ -15,6 +15,8 @@
 ! to the problem's minimiser (pqdts_set_polish). The per-stage files keep the
 ! reference's stopping point; rank_*_final.dat holds the polished result. Its
 ! progress lines start with " polish", which pqdts.py's parse ignores.
+! PQDTS_POLISH_METHOD=1 with it selects the direct face solves of the polish
+! (pqdts_set_polish_method); 0, the default, its Newton-CG steps.
 module pqdts_capi
   use iso_c_binding
   implicit none
@@ -150,6 +152,11 @@ module pqdts_capi
       type(c_ptr), value :: ctx
       integer(c_int), value :: max_outer, fp32
       real(c_double), value :: cg_rtol, tol, budget
+    end function
+    integer(c_int) function pqdts_set_polish_method(ctx, method) bind(C, name="pqdts_set_polish_method")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: method
     end function
     integer(c_int) function pqdts_device_timers(ctx, on) bind(C, name="pqdts_device_timers")
       import :: c_int, c_ptr
@@ -374,7 +381,7 @@ program pqdts_amd_driver
   character(len=32) :: arg
   character(len=64) :: fname
   logical :: exists
-  integer :: u, it
+  integer :: u, it, pm
   real(c_double) :: conv
 
   t0 = wtime()
@@ -446,6 +453,11 @@ program pqdts_amd_driver
     read (arg, *) it
     if (it > 0) call check(pqdts_set_polish(ctx, int(it, c_int), 1e-6_c_double, 1e-10_c_double, &
                                              0.0_c_double, 0_c_int), "pqdts_set_polish")
+    call get_environment_variable("PQDTS_POLISH_METHOD", arg, status=pm)
+    if (pm == 0) then
+      read (arg, *) pm
+      call check(pqdts_set_polish_method(ctx, int(pm, c_int)), "pqdts_set_polish_method")
+    end if
   end if
   deallocate (Fr, Fc, Fv, Pr, Pc, Pv)
   allocate (X(M, N))
