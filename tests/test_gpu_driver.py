@@ -201,3 +201,22 @@ def test_driver_polish_env(tmp_path, cfg1):
     X = pqdts_py_final(str(tmp_path))[5]
     Xe, ok = qp.exact(cfg1.F, cfg1.P, cfg1.gamma, X0=X)
     assert ok and np.max(np.abs(X - Xe)) <= 1e-8
+
+
+def test_driver_polish_method_env(tmp_path, cfg1):
+    """PQDTS_POLISH_METHOD=1 beside PQDTS_POLISH: the direct face solves polish the
+    drop-in driver's result to the same certified minimiser (within 1e-8 of
+    oracle/qp.py's exact solution)."""
+    from oracle import qp
+    NF, NP = workloads.write_inputs(str(tmp_path), cfg1)
+    os.environ["PQDTS_POLISH"] = "40"
+    os.environ["PQDTS_POLISH_METHOD"] = "1"
+    try:
+        out = run(str(tmp_path), cfg1, NF, NP, maxiter=200, output=2)
+    finally:
+        del os.environ["PQDTS_POLISH"]
+        del os.environ["PQDTS_POLISH_METHOD"]
+    assert any(ln.startswith(" polished status   1") for ln in out.splitlines())
+    X = pqdts_py_final(str(tmp_path))[5]
+    Xe, ok = qp.exact(cfg1.F, cfg1.P, cfg1.gamma, X0=X)
+    assert ok and np.max(np.abs(X - Xe)) <= 1e-8
