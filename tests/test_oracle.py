@@ -449,3 +449,24 @@ def test_qp_face_polish_globalised_matches_the_exact_solution():
         assert ok and np.max(np.abs(X - Xe)) <= 1e-12, p.name
         k = qp.kkt(p.F, p.P, p.gamma, X)
         assert k["x_min"] >= 0.0 and k["rowsum_err"] <= 1e-15 and k["stationarity"] <= 1e-12, (p.name, k)
+
+
+def test_qp_face_polish_at_a_vertex_minimiser():
+    """Noise-free P = F Pi_true with one-hot rows and gamma = 0: the gradient vanishes
+    at the minimiser, so every entry at 0 has mu = 0 to rounding. The rule's margin
+    (theta d) keeps those entries active instead of flipping with the rounding: the
+    restated iteration certifies Pi_true in a few iterations; without the margin it
+    does not within 60."""
+    from oracle import qp
+    rng = np.random.default_rng(8)
+    M, N, D = 12, 5, 30
+    F = sps.csr_matrix(rng.random((D, M)) + 0.1)
+    pi = np.zeros((M, N))
+    pi[np.arange(M), rng.integers(0, N, M)] = 1.0
+    P = np.asarray(F @ pi)
+    X0 = pi + 1e-7 * rng.random((M, N))
+    X0 /= X0.sum(1, keepdims=True)
+    X, status, its = qp.face_polish(F, P, 0.0, X0, with_iterations=True)
+    assert status == 1 and its <= 5 and np.max(np.abs(X - pi)) <= 1e-12
+    _, status0 = qp.face_polish(F, P, 0.0, X0, theta=0.0)
+    assert status0 == 2
