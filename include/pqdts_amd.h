@@ -178,7 +178,7 @@ int pqdts_set_polish(pqdts_ctx *ctx, int max_outer, double cg_rtol, double tol, 
  * (status 1: the unique minimiser), or with status 2 (X unchanged: the
  * minimisation's result) after max_outer iterations, when M > 3072 or a column
  * has more than 1024 free rows, or when the factorisation fails. cg_rtol, budget
- * and fp32 do not apply. Its buffers, about 8 M N + 8 M^2 + 8 sum_n m_n^2 bytes
+ * and fp32 do not apply. Its buffers, about 8 M N + 8 M^2 + 16 sum_n m_n^2 bytes
  * (m_n the free rows of column n), are held while the polish runs and are not
  * part of pqdts_footprint. */
 int pqdts_set_polish_method(pqdts_ctx *ctx, int method);
