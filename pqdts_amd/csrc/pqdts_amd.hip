@@ -273,9 +273,20 @@ struct Engine {
   Face *hface = nullptr;  // pinned host copy of dfaces (cap entries)
   std::vector<int64_t> fkcap, fkcap_old;
   // Stream-ordered allocation (hipMallocAsync) for the polish's buffers where the
-  // device supports memory pools: hipFree waits for the whole device, which would
-  // stall the other lanes' streams every time a problem enters or leaves the polish.
+  // device supports memory pools. Elsewhere (MI355X on ROCm 7.2 reports no pool
+  // support) the buffers come from hipMalloc at admission, and those the polish
+  // gives back are kept on a list and freed when the engine is idle or destroyed,
+  // or once kDeferMax have gathered: hipFree waits for the whole device, so freeing
+  // them while the engine runs would stall every other lane's stream.
   bool pools = false;
+  std::vector<void *> deferred;
+  static constexpr size_t kDeferMax = 4096;
+  void release_deferred() {
+    if (deferred.empty()) return;
+    (void)hipStreamSynchronize(s);
+    for (void *q : deferred) (void)hipFree(q);
+    deferred.clear();
+  }
   template <class T>
   T *salloc(int64_t n) {
     void *p = nullptr;
@@ -291,7 +302,11 @@ struct Engine {
     if (pools)
       (void)hipFreeAsync(p, s);
     else
-      (void)hipFree(p);
+      deferred.push_back(p);
+  }
+  // frees the deferred buffers once the engine is idle or too many have gathered
+  void settle_deferred() {
+    if (deferred.size() > kDeferMax || !any_active()) release_deferred();
   }
   // timers: host wall clock around the (asynchronous) launches, or with
   // dev_timers the device time between HIP events recorded on the stream at the
@@ -506,6 +521,20 @@ struct Engine {
     const int64_t c2 = need + need / 4 + 1024;
     f.kb = salloc<double>(c2);
     fkcap[p] = c2;
+  }
+  // A problem admitted with the direct polish gets its polish buffers now, with block
+  // storage for 4 M N doubles in each of the two generations, so that the polish
+  // itself allocates only when a problem needs more.
+  void face_prepare(int p) {
+    const pqdts_ctx *c = ctx[p];
+    if (c->pol.method != 1 || c->pol.max_outer <= 0 || c->M > FACE_MAX_ROWS) return;
+    face_alloc(p);
+    const int64_t r = 4 * c->MN;
+    face_grow(p, r);
+    face_swap(p);
+    face_grow(p, r);
+    face_swap(p);
+    face_upload_all();
   }
   void face_step(const std::vector<int> &QD, std::vector<int> &Fz);
 
@@ -1080,6 +1109,7 @@ Engine::~Engine() {
   }
   for (hipEvent_t e : kpool) (void)hipEventDestroy(e);
   for (int p = 0; p < cap; p++) face_free(p);
+  release_deferred();
   if (s) (void)hipStreamSynchronize(s);
   if (hface) (void)hipHostFree(hface);
   if (dfaces) (void)hipFree(dfaces);
@@ -1212,7 +1242,10 @@ void Engine::face_step(const std::vector<int> &QD, std::vector<int> &Fz) {
       st[p]->pol_status = 2;
       Fz.push_back(p);
     } else {
-      face_alloc(p);
+      if (hface[p].pos)
+        HIPC(hipMemsetAsync(hface[p].cdirty, 1, (size_t)ctx[p]->N, s));
+      else
+        face_alloc(p);
       init.push_back(p);
     }
   }
@@ -2271,6 +2304,7 @@ int engine_admit(Engine &e, pqdts_ctx *ctx, int mode, int maxiter, double eps, i
   ctx->pol_out = pqdts_ctx::PolishOut{};
   e.st[p].reset(q);
   ctx->full = true;
+  e.face_prepare(p);
   return p;
 }
 }  // namespace
@@ -2544,6 +2578,7 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
     q->pol = ctx->pol;
     ctx->pol_out = pqdts_ctx::PolishOut{};
     e.fp32[0] = 0;
+    e.face_prepare(0);
     Engine::Scope sc(&e, "min");
     // the reference starts in full space (pqdts.f90:1225)
     ctx->full = true;
@@ -2556,6 +2591,7 @@ int pqdts_solve(pqdts_ctx *ctx, int mode, int maxiter, double eps, int start_sta
     HIPC(hipStreamSynchronize(e.s));
     if (conv) *conv = q->conv;
     e.st[0].reset();
+    e.release_deferred();
   });
 }
 
@@ -2887,6 +2923,7 @@ int pqdts_batch_step(pqdts_batch *b, int *finished, int cap) {
     Engine &e = b->eng;
     if (e.any_active()) e.tick();
     HIPC(hipStreamSynchronize(e.s));
+    e.settle_deferred();
     for (int p = 0; p < e.cap; p++)
       if (e.ctx[p] && e.st[p] && e.st[p]->phase == 2) {
         if (finished && n < cap) finished[n] = p;

@@ -175,6 +175,7 @@ void server_lane(pqdts_server *S, int g) {
       live.erase(p);
       server_finish(S, r, rc2, msg);
     }
+    if (live.empty() || e.deferred.size() > Engine::kDeferMax) e.release_deferred();
   }
 }
 
